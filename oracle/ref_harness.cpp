@@ -15,7 +15,7 @@
 // in.bin : int32 nx, ny, nz, has_fields, has_force; then Q (5V doubles),
 //          [rho, ux, uy, uz (4V)], [fx, fy, fz (3V)]
 // out.bin: rho, ux, uy, uz (4V), Q (5V), F (3V), f (15V, i*V + cell),
-//          free energy (1), SimIO::Log sums (8)
+//          free energy (1), SimIO::Log sums (8); "-" writes nothing (timing)
 
 #include <chrono>
 #include <cstdio>
@@ -135,6 +135,7 @@ int run(const std::string& mode, int force_model, int steps, const char* in, con
     std::fprintf(stderr, "ref: %d steps of %zu cells in %.3f ms (%.3f MLUPS, %d threads)\n", steps, V, ms,
                  steps > 0 ? (double)V * steps / (ms * 1e3) : 0.0, Params::kNumOMPThreads);
 
+    if (std::string(out) == "-") return 0;   // timing run: no output file
     const double fe = TotalNematicFreeEnergy<BC>(qf);
     // SimIO::Log sums without the I/O (src/sim_io.cc:78-109), previous velocity zero.
     double sums[8] = {0, 0, 0, 0, 0, 0, 0, 0};
