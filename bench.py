@@ -1,0 +1,195 @@
+"""Throughput of the lbman3d step on one MI355X.
+
+Default workload (the bandwidth run): 256^3 FullyPeriodicConfig, FP32 with
+shifted populations, upwind advection, zeta = 0.046 (2x the params.h
+default), lambda = 0.7, seeded random director with S0 = 0.5 +- 0.01, rho = 1,
+u = 0. A "step" is one full coupled step (Q update + body force + D3Q15
+collide-stream) over the whole grid; inputs are resident in HBM before the
+timed region starts.
+
+Prints ONE JSON line: MLUPS (whole job), ms per step, the roofline of the
+dominant kernel (algorithmic bytes per launch / mean launch duration measured
+with HIP events on the solver's stream, against the 8 TB/s HBM peak), and a
+CPU baseline: the reference's own OpenMP CPU solver (oracle/_ref/ref_c2,
+compiled from its sources) timed on a few steps of the same workload on this
+host's cores.
+
+usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+# Algorithmic HBM bytes per lattice update: populations 15 read + 15 write,
+# Q 5 + 5, u 3 + 3 = 46 values (FP32: 184 B, FP64: 368 B). For the kernels of
+# the two-kernel split, the compulsory traffic of each kernel on its own.
+ALGO_BYTES = {
+    "fused_step_f32": 46 * 4,
+    "qstep_f32": (8 + 16) * 4,          # Q, u in; Q', Sigma, Tau, F out
+    "force_lbm_f32": (5 + 8 + 3 + 15 + 15 + 3) * 4,
+    "qstep_f64": (8 + 16) * 8,
+    "force_lbm_f64": (5 + 8 + 3 + 15 + 15 + 3 + 1 + 3) * 8,
+}
+STEP_BYTES = {"fp32": 46 * 4, "fp64": 46 * 8}
+PROFILE_DIR = os.path.join(ROOT, "profiles")
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def make_cfg(n, precision, upwind=True):
+    from lbman3d_amd import abi
+    return abi.make_config(n, n, n, abi.FullyPeriodicConfig, abi.Params(ALPHA=0.046, LAMBDA=0.7),
+                           advection=abi.ADV_UPWIND if upwind else abi.ADV_CENTRED, precision=precision)
+
+
+def time_solver(cfg, q, steps, warmup, schedule=None, autotune=False):
+    from lbman3d_amd.solver import DeviceSolver
+    s = DeviceSolver(cfg)
+    s.upload_state(q=q)
+    tuned = None
+    if autotune:
+        tuned = s.autotune(5)
+    if schedule is not None:
+        s.set_schedule(schedule)
+    s.step(warmup)
+    s.synchronize()
+    t0 = time.perf_counter()
+    ev_ms = s.time_steps(steps)
+    wall = time.perf_counter() - t0
+    prof = s.profile_kernels(min(20, steps))
+    diag = s.diagnostics()
+    return s, ev_ms, wall, prof, diag, tuned
+
+
+def roofline(prof, V):
+    name = max(prof, key=lambda k: prof[k])
+    ms = prof[name]
+    algo = ALGO_BYTES.get(name)
+    if algo is None:
+        return name, None
+    achieved = algo * V / (ms * 1e-3) / 1e9
+    traffic = None
+    pmc = os.path.join(PROFILE_DIR, f"{name}_256cube_traffic.json")
+    if os.path.exists(pmc):
+        with open(pmc) as fp:
+            traffic = json.load(fp).get("hbm_bytes_per_launch")
+    return name, {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                  "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": name,
+                  "kernel_ms": round(ms, 4), "algorithmic_bytes_per_launch": algo * V}
+
+
+def cpu_baseline(q, n, steps=4):
+    """The reference's OpenMP CPU solver on `steps` steps of the same 256^3
+    workload (oracle/_ref/ref_c2: FullyPeriodic, upwind, zeta 0.046, lambda
+    0.7, 16 OpenMP threads), timed by its own step loop."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "ref_c2")
+    if not os.path.exists(exe) or n != 256:
+        return None
+    with tempfile.TemporaryDirectory(prefix="lbmcpu_") as td:
+        fin = os.path.join(td, "in.bin")
+        with open(fin, "wb") as fp:
+            np.array([n, n, n, 0, 0], dtype=np.int32).tofile(fp)
+            np.ascontiguousarray(q, dtype=np.float64).reshape(-1).tofile(fp)
+        env = dict(os.environ, OMP_PROC_BIND="close", OMP_PLACES="cores")
+        r = subprocess.run([exe, "FullyPeriodic", "full", "0", str(steps), fin, "-"], capture_output=True,
+                           text=True, env=env, timeout=600)
+    m = re.search(r"ref: (\d+) steps of (\d+) cells in ([0-9.]+) ms \(([0-9.]+) MLUPS, (\d+) threads\)", r.stderr)
+    if r.returncode != 0 or not m:
+        log(f"cpu baseline failed: {r.stderr[-400:]}")
+        return None
+    return {"value": float(m.group(4)), "unit": "MLUPS", "cores": int(m.group(5)), "kind": "reference",
+            "sample": f"{m.group(1)} steps of the 256^3 FP64 workload (reference CPU solver, "
+                      f"{m.group(5)} OpenMP threads, {m.group(3)} ms)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--n", type=int, default=256, help="grid edge")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--schedule", type=int, default=None, help="force an FP32 kernel schedule")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU: this build has no multi-GPU path (requested --gpus {args.gpus})",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from lbman3d_amd import abi
+    from lbman3d_amd.solver import seed_director_field, tile_count, device_info
+    n = args.n
+    V = n * n * n
+    log(device_info())
+    log(f"seeding {n}^3 director field")
+    q = seed_director_field(2024, n, n, n, 0.5, 0.01)
+
+    autotune = args.schedule is None and tile_count() > 0
+    log("timing FP32 shifted step")
+    s, ev_ms, wall, prof, diag, tuned = time_solver(make_cfg(n, abi.FP32_SHIFTED), q, args.steps, args.warmup,
+                                                    schedule=args.schedule, autotune=autotune)
+    schedule = s.get_schedule()
+    s.close()
+    ms_step = ev_ms / args.steps
+    mlups = V / (ms_step * 1e-3) / 1e6
+    kname, roof = roofline(prof, V)
+    step_frac = STEP_BYTES["fp32"] * V / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS
+    log(f"FP32: {ms_step:.4f} ms/step, {mlups:.1f} MLUPS, kernels {prof}")
+
+    secondary = {"fp32_step_hbm_frac": round(step_frac, 4), "fp32_kernels_ms": prof,
+                 "fp32_wall_ms_per_step": round(wall * 1e3 / args.steps, 4),
+                 "fp32_umax": diag["umax"], "fp32_nonfinite": diag["nonfinite"]}
+    if tuned:
+        secondary["autotune_ms_per_step"] = tuned
+    if not args.no_fp64:
+        log("timing FP64 plain step")
+        k64 = max(10, args.steps // 5)
+        s64, ev64, _, prof64, _, _ = time_solver(make_cfg(n, abi.FP64), q, k64, 5)
+        s64.close()
+        ms64 = ev64 / k64
+        secondary.update({"fp64_ms_per_step": round(ms64, 4), "fp64_mlups": round(V / (ms64 * 1e-3) / 1e6, 1),
+                          "fp64_step_hbm_frac": round(STEP_BYTES["fp64"] * V / (ms64 * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                          "fp64_kernels_ms": prof64})
+
+    cpu = None if args.no_cpu_baseline else cpu_baseline(q, n)
+    out = {
+        "metric": "MLUPS and % peak HBM BW; FP32/batched/closure vs FP64 plain at equal error",
+        "value": round(mlups, 1),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic: seeded random director field, rho=1, u=0",
+        "config": {"workload": f"{n}^3 FullyPeriodicConfig coupled LB+Q-tensor step, FP32 shifted populations, "
+                               "upwind advection, zeta=0.046, lambda=0.7, S0=0.5+-0.01",
+                   "grid": [n, n, n], "schedule": schedule, "parallelism": "single GPU"},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "secondary": secondary,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -10,5 +10,4 @@ int lbm_batch_download_state(lbm_batch_handle, int32_t, double*, double*, double
 int lbm_batch_step(lbm_batch_handle, int32_t, int32_t) { return fail(LBM_ERR_UNSUPPORTED, "batched form not built"); }
 int lbm_batch_time_steps(lbm_batch_handle, int32_t, int32_t, double*) { return fail(LBM_ERR_UNSUPPORTED, "batched form not built"); }
 int lbm_batch_diagnostics(lbm_batch_handle, int32_t, lbm_diag*) { return fail(LBM_ERR_UNSUPPORTED, "batched form not built"); }
-int lbm_autotune(lbm_handle, int32_t, double*, int32_t, int32_t* count) { if (count) *count = 0; return fail(LBM_ERR_UNSUPPORTED, "autotuner not built"); }
 }

@@ -240,7 +240,8 @@ struct lbm_solver {
     int diag_blocks = 0;
     int pulled = 0;        // 0: g holds post-streaming f; 1: g holds post-collision
     int schedule = 0;      // FP32: 0 auto, 1 two-kernel, 2+k fused tile k
-    int tuned_tile = -1;   // autotuned fused tile (auto schedule)
+    int tuned = 0;         // schedule picked by lbm_autotune for "auto" (0: not tuned)
+    bool fused_ok = false; // fused kernel usable (FP32, 32-bit offsets fit)
     bool moving = false;
 
     size_t esize() const { return fp64 ? sizeof(double) : sizeof(float); }
@@ -275,6 +276,8 @@ int alloc_fields(lbm_solver* s, Fields<T>& f) {
         f.F[m] = p;
         if (int r = get(fb, (void**)&p)) return r;
         f.u[m] = p;
+        if (int r = get(fb, (void**)&p)) return r;
+        f.un[m] = p;
     }
     if (int r = get(fb, (void**)&p)) return r;
     f.rho = p;
@@ -347,6 +350,10 @@ template <typename T>
 void swap_g(Fields<T>& f) {
     std::swap(f.g, f.gn);
 }
+template <typename T>
+void swap_u(Fields<T>& f) {
+    for (int m = 0; m < 3; ++m) std::swap(f.u[m], f.un[m]);
+}
 
 }  // namespace
 
@@ -399,11 +406,11 @@ int run_lbm(lbm_solver* s, KernelRecorder* rec) {
     return LBM_OK;
 }
 
+// Fused tile index for the next step, or -1 for the two-kernel split.
 int active_tile(const lbm_solver* s) {
-    if (s->fp64) return -1;
-    if (s->schedule == 1) return -1;
-    if (s->schedule >= 2) return s->schedule - 2;
-    return s->tuned_tile;
+    if (s->fp64 || !s->fused_ok) return -1;
+    int sched = s->schedule ? s->schedule : (s->tuned ? s->tuned : 2);
+    return sched >= 2 ? sched - 2 : -1;
 }
 
 // One full step. last: this is the final step of a batch of steps, so the
@@ -429,6 +436,7 @@ int run_step(lbm_solver* s, bool last, KernelRecorder* rec) {
             if (rec) rec->end();
             swap_q(s->d32);
             swap_g(s->d32);
+            swap_u(s->d32);
         } else {
             if (rec) rec->begin("qstep_f32");
             launch_qstep(s->d32, s->bc32, s->k32, s->V, st);
@@ -526,6 +534,8 @@ int lbm_create(const lbm_config* cfg, lbm_handle* out) {
     s->vs = (s->V + 63) / 64 * 64;
     s->fp64 = cfg->precision == LBM_FP64;
     s->moving = has_moving_wall(*cfg);
+    // The fused kernel addresses populations with 32-bit byte offsets.
+    s->fused_ok = !s->fp64 && fused_tile_count() > 0 && (double)kQ * s->vs * sizeof(float) < 4294967296.0;
     auto cleanup = [&](int r) {
         lbm_destroy(s);
         return r;
@@ -857,6 +867,74 @@ int lbm_tile_name(int32_t schedule, char* buf, size_t len) {
     else return fail(LBM_ERR_INVALID, "schedule out of range");
     std::strncpy(buf, n.c_str(), len - 1);
     buf[len - 1] = 0;
+    return LBM_OK;
+}
+
+int lbm_autotune(lbm_handle s, int32_t n, double* ms_out, int32_t max, int32_t* count) {
+    if (!s || n < 1) return fail(LBM_ERR_INVALID, "bad argument");
+    if (count) *count = 0;
+    if (s->fp64) return fail(LBM_ERR_UNSUPPORTED, "the autotuner selects FP32 kernel schedules");
+    // Snapshot the evolving state so tuning leaves the simulation untouched.
+    Fields<float>& f = s->d32;
+    const size_t fb = s->V * sizeof(float), gb = kQ * s->vs * sizeof(float);
+    std::vector<std::pair<void*, size_t>> parts;
+    for (int m = 0; m < 5; ++m) parts.push_back({f.q[m], fb});
+    for (int m = 0; m < 3; ++m) parts.push_back({f.u[m], fb});
+    for (int m = 0; m < 3; ++m) parts.push_back({f.F[m], fb});
+    parts.push_back({f.rho, fb});
+    size_t total = gb;
+    for (auto& p : parts) total += p.second;
+    void* snap = nullptr;
+    HIPCHK(hipMalloc(&snap, total));
+    auto copy_all = [&](bool save) -> int {
+        char* cur = (char*)snap;
+        HIPCHK(hipMemcpyAsync(save ? cur : (void*)f.g, save ? (void*)f.g : cur, gb, hipMemcpyDeviceToDevice, s->stream));
+        cur += gb;
+        for (auto& p : parts) {
+            HIPCHK(hipMemcpyAsync(save ? cur : p.first, save ? p.first : cur, p.second, hipMemcpyDeviceToDevice, s->stream));
+            cur += p.second;
+        }
+        return LBM_OK;
+    };
+    // The field pointers above are the current ones; snapshots restore into them.
+    const int pulled0 = s->pulled, sched0 = s->schedule;
+    int r = copy_all(true);
+    std::vector<int> cands = {1};
+    if (s->fused_ok)
+        for (int t = 0; t < fused_tile_count(); ++t) cands.push_back(2 + t);
+    double best = 1e300;
+    int best_s = 1, k = 0;
+    for (int c : cands) {
+        if (r) break;
+        // Restore pointers to the snapshot's owners: swaps during timing
+        // permute buffers, so rebuild the current set from the snapshot.
+        s->schedule = c;
+        s->pulled = pulled0;
+        double ms = 0.0;
+        if ((r = run_step(s, false, nullptr))) break;
+        if ((r = lbm_time_steps(s, n, &ms))) break;
+        ms /= n;
+        if (k < max && ms_out) ms_out[k] = ms;
+        ++k;
+        if (ms < best) { best = ms; best_s = c; }
+        // Put the original state back (into whatever buffers are now current).
+        Fields<float> now = s->d32;
+        char* cur = (char*)snap;
+        if (hipMemcpyAsync(now.g, cur, gb, hipMemcpyDeviceToDevice, s->stream) != hipSuccess) { r = fail(LBM_ERR_HIP, "restore failed"); break; }
+        cur += gb;
+        void* dst[12] = {now.q[0], now.q[1], now.q[2], now.q[3], now.q[4], now.u[0], now.u[1], now.u[2], now.F[0], now.F[1], now.F[2], now.rho};
+        for (int i = 0; i < 12 && !r; ++i) {
+            if (hipMemcpyAsync(dst[i], cur, parts[i].second, hipMemcpyDeviceToDevice, s->stream) != hipSuccess) r = fail(LBM_ERR_HIP, "restore failed");
+            cur += parts[i].second;
+        }
+    }
+    (void)hipStreamSynchronize(s->stream);
+    (void)hipFree(snap);
+    s->pulled = pulled0;
+    s->schedule = sched0;
+    if (r) return r;
+    s->tuned = best_s;
+    if (count) *count = std::min(k, (int)max);
     return LBM_OK;
 }
 

@@ -90,7 +90,8 @@ struct Fields {
     T* sig[5];      // Sigma^n
     T* tau[3];      // Tau^n (xy, xz, yz)
     T* F[3];        // body force
-    T* u[3];        // velocity (in place: u^{n-1} -> u^n)
+    T* u[3];        // velocity u^{n-1} (unfused kernels update it in place)
+    T* un[3];       // velocity u^n written by the fused kernel
     T* rho;         // density
     T* g;           // populations, source buffer   [15][vs]
     T* gn;          // populations, destination     [15][vs]

@@ -1,9 +1,479 @@
-// fused_step.hpp — placeholder until the fused kernel lands.
+// fused_step.hpp — the whole coupled step in one FP32 kernel.
+//
+// Per output cell the step needs Q^{n+1}, Sigma^n and Tau^n at the cell and
+// its six neighbours (body force: grad Q^{n+1} and div(Sigma + Tau),
+// src/qtensor_solver.tpp:154-217), which in turn need Q^n and u^{n-1} two
+// cells away. Instead of writing Q^{n+1}, Sigma, Tau and the Ericksen force
+// to HBM and reading them back (the reference's three-kernel data flow), a
+// block owns a TX x TY column tile, marches up a chunk of z planes, and keeps
+//   * the current input plane (Q^n, u^{n-1}) of the (TX+4) x (TY+4) region
+//     in LDS for xy neighbours, with z neighbours in registers;
+//   * the Q-update results (Q^{n+1}, Sigma, Tau) of the (TX+2) x (TY+2)
+//     region for the last two planes in LDS;
+// and finishes each plane with the force, pull gather and BGK collision of
+// the interior cells. HBM traffic per cell is then the compulsory state only:
+// populations 15 + 15, Q 5 + 5, u 3 + 3 values.
+//
+// Boundary handling is the unfused kernels' (physics.hpp, gather): every
+// column of the extended tile holds the values of the domain column that the
+// reference's index rule maps it to (periodic wrap or clamp, per face tag),
+// and neighbour reads of the force stage substitute the cell itself where
+// the reference clamps.
 #pragma once
+
 #include "launch.hpp"
+#include "physics.hpp"
+#include "step_kernels.hpp"
+
 namespace lbm {
-int fused_tile_count() { return 0; }
-const char* fused_tile_name(int) { return "none"; }
-void launch_fused_step_f32(const Fields<float>&, const BCDesc<float>&, const Phys<float>&, int, int, int,
-                           hipStream_t) {}
+
+// Map a coordinate of the extended tile into the domain: periodic side wraps,
+// any other side clamps (the rule of nb_index for arbitrary distance).
+__device__ __forceinline__ int map_coord(int t, int n, bool lo_per, bool hi_per) {
+    if (t < 0) return lo_per ? ((t % n) + n) % n : 0;
+    if (t >= n) return hi_per ? t % n : n - 1;
+    return t;
+}
+
+// Global access as uniform base + 32-bit byte offset, so the compiler can use
+// the scalar-base addressing form (one VGPR per address instead of two).
+__device__ __forceinline__ float ldo(const float* base, unsigned boff) {
+    return *reinterpret_cast<const float*>(reinterpret_cast<const char*>(base) + boff);
+}
+__device__ __forceinline__ void sto(float* base, unsigned boff, float v) {
+    *reinterpret_cast<float*>(reinterpret_cast<char*>(base) + boff) = v;
+}
+
+// Pull gather with 32-bit offsets (fused kernel; requires 15 * vs * 4 < 2^32).
+template <bool PERIODIC>
+__device__ __forceinline__ void gather32(const Fields<float>& f, const BCDesc<float>& bc, int x, int y, int z,
+                                         unsigned c, int pull, float out[kQ]) {
+    const int nx = bc.n[0], ny = bc.n[1], nz = bc.n[2];
+    const unsigned vs4 = (unsigned)f.vs * 4u;
+    const float* g = f.g;
+    if (!pull) {
+#pragma unroll
+        for (int j = 0; j < kQ; ++j) out[j] = ldo(g, j * vs4 + c * 4u);
+        return;
+    }
+    const int cls = node_class(x, y, z, nx, ny, nz);
+    if (PERIODIC || cls == 0) {
+        const int xm = x == 0 ? nx - 1 : x - 1, xp = x == nx - 1 ? 0 : x + 1;
+        const int ym = y == 0 ? ny - 1 : y - 1, yp = y == ny - 1 ? 0 : y + 1;
+        const int zm = z == 0 ? nz - 1 : z - 1, zp = z == nz - 1 ? 0 : z + 1;
+        const int xs[3] = {xp, x, xm}, ys[3] = {yp, y, ym}, zs[3] = {zp, z, zm};
+#pragma unroll
+        for (int j = 0; j < kQ; ++j) {
+            const unsigned src = ((unsigned)zs[cz(j) + 1] * ny + ys[cy(j) + 1]) * nx + xs[cx(j) + 1];
+            out[j] = ldo(g, j * vs4 + src * 4u);
+        }
+        return;
+    }
+    const uint16_t* row = f.slots + cls * kQ;
+    float rho_prev = 0.0f;
+    bool have_rho = false;
+#pragma unroll
+    for (int j = 0; j < kQ; ++j) {
+        const uint16_t e = row[j];
+        const int kind = slot_kind(e);
+        if (kind == SLOT_STREAM) {
+            int xs = x - cx(j), ys = y - cy(j), zs = z - cz(j);
+            xs = xs < 0 ? xs + nx : (xs >= nx ? xs - nx : xs);
+            ys = ys < 0 ? ys + ny : (ys >= ny ? ys - ny : ys);
+            zs = zs < 0 ? zs + nz : (zs >= nz ? zs - nz : zs);
+            out[j] = ldo(g, j * vs4 + (((unsigned)zs * ny + ys) * nx + xs) * 4u);
+        } else if (kind == SLOT_MIRROR) {
+            out[j] = ldo(g, slot_src(e) * vs4 + c * 4u);
+        } else {
+            const int face = slot_face(e);
+            if (!have_rho) { rho_prev = ldo(f.rho, c * 4u); have_rho = true; }
+            const float dot = float(cx(j)) * bc.wall_u[face][0] + float(cy(j)) * bc.wall_u[face][1]
+                            + float(cz(j)) * bc.wall_u[face][2];
+            out[j] = ldo(g, slot_src(e) * vs4 + c * 4u) + 6.0f * rho_prev * float(wgt(j)) * dot;
+        }
+    }
+}
+
+template <int TX, int TY>
+struct TileGeom {
+    static constexpr int NT = TX * TY;               // threads = interior columns
+    static constexpr int AX = TX + 2, AY = TY + 2;   // Q-update region
+    static constexpr int IX = TX + 4, IY = TY + 4;   // input region
+    static constexpr int NA = AX * AY, NI = IX * IY;
+    static constexpr int NR = 2 * AX + 2 * (AY - 2); // Q-update ring columns
+    static constexpr int NO = 2 * IX + 2 * (IY - 2); // input outer ring columns
+    static_assert(NR <= NT && NO <= NT, "tile too small for its halo");
+};
+
+// Border column r of a W x H rectangle (top row, bottom row, left, right).
+__device__ __forceinline__ void ring_col(int r, int W, int H, int& a, int& b) {
+    if (r < W) { a = r; b = 0; }
+    else if (r < 2 * W) { a = r - W; b = H - 1; }
+    else if (r < 2 * W + (H - 2)) { a = 0; b = 1 + r - 2 * W; }
+    else { a = W - 1; b = 1 + r - 2 * W - (H - 2); }
+}
+
+struct AxisTags {
+    bool qlo, qhi, ulo, uhi;   // periodic flags per side for Q and velocity tags
+};
+
+// Q-update (stage A) results kept per plane.
+struct StageA {
+    float qn[5], sg[5], ta[3];
+};
+
+constexpr int kNumIn = 8;    // Q (5) + u (3)
+constexpr int kNumA = 13;    // Q^{n+1} (5) + Sigma (5) + Tau (3)
+
+template <int TX, int TY>
+__device__ __forceinline__ void load_column(const Fields<float>& f, int qcol, int ucol_xy, unsigned qplane,
+                                            unsigned uplane, float v[kNumIn]) {
+    const unsigned qo = (qplane + (unsigned)qcol) * 4u, uo = (uplane + (unsigned)ucol_xy) * 4u;
+#pragma unroll
+    for (int m = 0; m < 5; ++m) v[m] = ldo(f.q[m], qo);
+#pragma unroll
+    for (int m = 0; m < 3; ++m) v[5 + m] = ldo(f.u[m], uo);
+}
+
+// Stage A at one column: Beris-Edwards update from the input plane in LDS
+// (xy neighbours) and the column's own z neighbours.
+template <int TX, int TY>
+__device__ __forceinline__ void stage_a(const BCDesc<float>& bc, const Phys<float>& k,
+                                        const float (*sIn)[TileGeom<TX, TY>::NI], int ii, int jj,
+                                        const float zm[kNumIn], const float c[kNumIn], const float zp[kNumIn],
+                                        const int qpos[3], const int upos[3], StageA& out, float fe[3]) {
+    using G = TileGeom<TX, TY>;
+    const int cc = jj * G::IX + ii;
+    const int xm = cc - 1, xp = cc + 1, ym = cc - G::IX, yp = cc + G::IX;
+    float Q[5];
+    Deriv<float> dq[5];
+#pragma unroll
+    for (int m = 0; m < 5; ++m) {
+        Q[m] = c[m];
+        dq[m] = q_deriv(bc, m, qpos, c[m], sIn[m][xm], sIn[m][xp], sIn[m][ym], sIn[m][yp], zm[m], zp[m]);
+    }
+    float u[3], g[3][3];
+#pragma unroll
+    for (int A = 0; A < 3; ++A) {
+        const float v0 = c[5 + A];
+        u[A] = v0;
+        g[A][0] = u_grad(bc, 0, A, upos[0], sIn[5 + A][xm], v0, sIn[5 + A][xp]);
+        g[A][1] = u_grad(bc, 1, A, upos[1], sIn[5 + A][ym], v0, sIn[5 + A][yp]);
+        g[A][2] = u_grad(bc, 2, A, upos[2], zm[5 + A], v0, zp[5 + A]);
+    }
+    q_pointwise(k, Q, u, dq, g, out.qn, out.sg, out.ta, fe);
+}
+
+// Components of the stage-A results whose z differences the force needs:
+// Q^{n+1} xx, xz, yy, yz; Sigma xx, xz, yy, yz; Tau xz, yz.
+constexpr int kNumDZ = 10;
+__device__ __forceinline__ int dz_slot(int m) {
+    switch (m) {
+        case 0: return 0;
+        case 2: return 1;
+        case 3: return 2;
+        case 4: return 3;
+        case 5: return 4;
+        case 7: return 5;
+        case 8: return 6;
+        case 9: return 7;
+        case 11: return 8;
+        case 12: return 9;
+        default: return -1;
+    }
+}
+__device__ __forceinline__ float stage_value(const StageA& s, int m) {
+    return m < 5 ? s.qn[m] : (m < 10 ? s.sg[m - 5] : s.ta[m - 10]);
+}
+
+template <int TX, int TY, int ZC, bool PERIODIC>
+__global__ void __launch_bounds__(TX * TY)
+fused_step_kernel(Fields<float> f, BCDesc<float> bc, Phys<float> k, int pull, int aux, int ntx, int nty,
+                  int nzc) {
+    using G = TileGeom<TX, TY>;
+    __shared__ float sIn[kNumIn][G::NI];          // input plane pa, extended region
+    __shared__ float sA[2][kNumA][G::NA];         // stage-A results, planes pa and pa - 1
+    __shared__ float sRing[3][kNumIn][G::NR];     // input history of the stage-A ring columns
+
+    // XCD-aware block order: hardware deals consecutive block ids round-robin
+    // over the 8 XCDs; renumber so each XCD gets a contiguous run of tiles
+    // (neighbouring tiles share halos in that XCD's L2).
+    const int nblk = ntx * nty * nzc;
+    const int per_xcd = (nblk + 7) / 8;
+    const int L = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
+    if (L >= nblk) return;
+    const int tx = L % ntx, ty = (L / ntx) % nty, tz = L / (ntx * nty);
+    const int nx = bc.n[0], ny = bc.n[1], nz = bc.n[2];
+    const int x0 = tx * TX, y0 = ty * TY;
+    const int z0 = tz * ZC, z1 = min(z0 + ZC, nz);
+    const unsigned plane = (unsigned)nx * ny;
+
+    // Periodic flags per face; all true in the fully periodic instantiation,
+    // which then compiles without any ghost, clamp or wall-slot code.
+    AxisTags ax[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        ax[a].qlo = PERIODIC || bc.qkind[2 * a] == QB_PERIODIC;
+        ax[a].qhi = PERIODIC || bc.qkind[2 * a + 1] == QB_PERIODIC;
+        ax[a].ulo = PERIODIC || bc.ukind[2 * a] == UB_PERIODIC;
+        ax[a].uhi = PERIODIC || bc.ukind[2 * a + 1] == UB_PERIODIC;
+    }
+
+    const int t = threadIdx.x;
+    // Interior column of this thread (stage-A region coordinates ia, ib).
+    const int ia = t % TX + 1, ib = t / TX + 1;
+    const int acell = ib * G::AX + ia;
+    const int gx = x0 + ia - 1, gy = y0 + ib - 1;
+    const bool active = gx < nx && gy < ny;
+    // Stage-A ring column (threads t < NR) and input outer-ring column (last NO threads).
+    const bool has_ring = t < G::NR;
+    int ra = 0, rb = 0;
+    if (has_ring) ring_col(t, G::AX, G::AY, ra, rb);
+    const int orank = t - (G::NT - G::NO);
+    const bool has_outer = orank >= 0;
+    int oi = 0, oj = 0;
+    if (has_outer) ring_col(orank, G::IX, G::IY, oi, oj);
+
+    // Domain columns (Q tags / velocity tags) of the owned columns.
+    const int qx0 = map_coord(gx, nx, ax[0].qlo, ax[0].qhi), qy0 = map_coord(gy, ny, ax[1].qlo, ax[1].qhi);
+    const int ux0 = map_coord(gx, nx, ax[0].ulo, ax[0].uhi), uy0 = map_coord(gy, ny, ax[1].ulo, ax[1].uhi);
+    const int rgx = x0 - 1 + ra, rgy = y0 - 1 + rb;
+    const int qxr = map_coord(rgx, nx, ax[0].qlo, ax[0].qhi), qyr = map_coord(rgy, ny, ax[1].qlo, ax[1].qhi);
+    const int uxr = map_coord(rgx, nx, ax[0].ulo, ax[0].uhi), uyr = map_coord(rgy, ny, ax[1].ulo, ax[1].uhi);
+    int oqc = 0, ouc = 0;
+    if (has_outer) {
+        const int ogx = x0 - 2 + oi, ogy = y0 - 2 + oj;
+        oqc = map_coord(ogy, ny, ax[1].qlo, ax[1].qhi) * nx + map_coord(ogx, nx, ax[0].qlo, ax[0].qhi);
+        ouc = map_coord(ogy, ny, ax[1].ulo, ax[1].uhi) * nx + map_coord(ogx, nx, ax[0].ulo, ax[0].uhi);
+    }
+    const int qc0 = qy0 * nx + qx0, uc0 = uy0 * nx + ux0;
+    const int qcr = qyr * nx + qxr, ucr = uyr * nx + uxr;
+
+    auto qplane = [&](int z) { return (unsigned)map_coord(z, nz, ax[2].qlo, ax[2].qhi) * plane; };
+    auto uplane = [&](int z) { return (unsigned)map_coord(z, nz, ax[2].ulo, ax[2].uhi) * plane; };
+
+    // Input history of the interior column: planes pa-1, pa, pa+1 (registers).
+    float hin[3][kNumIn];
+    load_column<TX, TY>(f, qc0, uc0, qplane(z0 - 2), uplane(z0 - 2), hin[1]);
+    load_column<TX, TY>(f, qc0, uc0, qplane(z0 - 1), uplane(z0 - 1), hin[2]);
+    if (has_ring) {
+        float v[kNumIn];
+        load_column<TX, TY>(f, qcr, ucr, qplane(z0 - 2), uplane(z0 - 2), v);
+#pragma unroll
+        for (int m = 0; m < kNumIn; ++m) sRing[(z0 - 2 + 6) % 3][m][t] = v[m];
+        load_column<TX, TY>(f, qcr, ucr, qplane(z0 - 1), uplane(z0 - 1), v);
+#pragma unroll
+        for (int m = 0; m < kNumIn; ++m) sRing[(z0 - 1 + 6) % 3][m][t] = v[m];
+    }
+
+    // z-difference values of the interior column's stage A at plane pa - 2
+    // (the lower z neighbour of the plane finished in step 3), and the
+    // Ericksen force of the interior column for planes pa - 1 and pa.
+    float dz_back[kNumDZ];
+#pragma unroll
+    for (int m = 0; m < kNumDZ; ++m) dz_back[m] = 0.0f;
+    float fe_prev[3] = {0, 0, 0}, fe_cur[3] = {0, 0, 0};
+
+    for (int pa = z0 - 1; pa <= z1; ++pa) {
+        // ---- step 1: input plane pa into LDS; history advance ----
+#pragma unroll
+        for (int m = 0; m < kNumIn; ++m) {
+            hin[0][m] = hin[1][m];
+            hin[1][m] = hin[2][m];
+        }
+        load_column<TX, TY>(f, qc0, uc0, qplane(pa + 1), uplane(pa + 1), hin[2]);
+        {
+            const int cc = (ib + 1) * G::IX + ia + 1;
+#pragma unroll
+            for (int m = 0; m < kNumIn; ++m) sIn[m][cc] = hin[1][m];
+        }
+        const int r_m = (pa - 1 + 6) % 3, r_c = (pa + 6) % 3, r_p = (pa + 1 + 6) % 3;
+        if (has_ring) {
+            float v[kNumIn];
+            load_column<TX, TY>(f, qcr, ucr, qplane(pa + 1), uplane(pa + 1), v);
+            const int cc = (rb + 1) * G::IX + ra + 1;
+#pragma unroll
+            for (int m = 0; m < kNumIn; ++m) {
+                sRing[r_p][m][t] = v[m];
+                sIn[m][cc] = sRing[r_c][m][t];
+            }
+        }
+        if (has_outer) {
+            const unsigned qo = (qplane(pa) + (unsigned)oqc) * 4u, uo = (uplane(pa) + (unsigned)ouc) * 4u;
+            const int cc = oj * G::IX + oi;
+#pragma unroll
+            for (int m = 0; m < 5; ++m) sIn[m][cc] = ldo(f.q[m], qo);
+#pragma unroll
+            for (int m = 0; m < 3; ++m) sIn[5 + m][cc] = ldo(f.u[m], uo);
+        }
+        __syncthreads();
+
+        // ---- step 2: Q update (stage A) at plane pa ----
+        const int par = (pa + 2) & 1;
+        const int zq = map_coord(pa, nz, ax[2].qlo, ax[2].qhi), zu = map_coord(pa, nz, ax[2].ulo, ax[2].uhi);
+        float dz_fwd[kNumDZ];
+        fe_prev[0] = fe_cur[0];
+        fe_prev[1] = fe_cur[1];
+        fe_prev[2] = fe_cur[2];
+        // Interior column (slot 0, inputs in registers) and, for the first
+        // NR threads, one ring column (slot 1, inputs in LDS); one copy of
+        // the Q-update code serves both.
+#pragma unroll 1
+        for (int slot = 0; slot < (has_ring ? 2 : 1); ++slot) {
+            float zm[kNumIn], c[kNumIn], zp[kNumIn];
+            int qpos[3], upos[3], ci, cj, acol;
+            if (slot == 0) {
+#pragma unroll
+                for (int m = 0; m < kNumIn; ++m) {
+                    zm[m] = hin[0][m];
+                    c[m] = hin[1][m];
+                    zp[m] = hin[2][m];
+                }
+                qpos[0] = qx0; qpos[1] = qy0; upos[0] = ux0; upos[1] = uy0;
+                ci = ia; cj = ib; acol = acell;
+            } else {
+#pragma unroll
+                for (int m = 0; m < kNumIn; ++m) {
+                    zm[m] = sRing[r_m][m][t];
+                    c[m] = sRing[r_c][m][t];
+                    zp[m] = sRing[r_p][m][t];
+                }
+                qpos[0] = qxr; qpos[1] = qyr; upos[0] = uxr; upos[1] = uyr;
+                ci = ra; cj = rb; acol = rb * G::AX + ra;
+            }
+            qpos[2] = zq;
+            upos[2] = zu;
+            StageA a;
+            float fe[3];
+            stage_a<TX, TY>(bc, k, sIn, ci + 1, cj + 1, zm, c, zp, qpos, upos, a, fe);
+#pragma unroll
+            for (int m = 0; m < kNumA; ++m) sA[par][m][acol] = stage_value(a, m);
+            if (slot == 0) {
+#pragma unroll
+                for (int m = 0; m < kNumA; ++m)
+                    if (dz_slot(m) >= 0) dz_fwd[dz_slot(m)] = stage_value(a, m);
+                fe_cur[0] = fe[0];
+                fe_cur[1] = fe[1];
+                fe_cur[2] = fe[2];
+            }
+        }
+        __syncthreads();
+
+        // ---- step 3: force + collide-stream at plane pb = pa - 1 ----
+        const int pb = pa - 1;
+        const float* A0 = &sA[(pb + 2) & 1][0][0];
+        auto A = [&](int m, int col) { return A0[m * G::NA + col]; };
+        if (pb >= z0 && active) {
+            // Neighbour columns with the reference's fetch rule: where a
+            // non-periodic face clamps, the cell itself.
+            const int lxm = (qx0 == 0 && !ax[0].qlo) ? acell : acell - 1;
+            const int lxp = (qx0 == nx - 1 && !ax[0].qhi) ? acell : acell + 1;
+            const int lym = (qy0 == 0 && !ax[1].qlo) ? acell : acell - G::AX;
+            const int lyp = (qy0 == ny - 1 && !ax[1].qhi) ? acell : acell + G::AX;
+            const bool zm_self = (pb == 0 && !ax[2].qlo);
+            const bool zp_self = (pb == nz - 1 && !ax[2].qhi);
+            auto zmv = [&](int m) -> float {
+                const int sl = dz_slot(m);
+                return (zm_self || sl < 0) ? A(m, acell) : dz_back[sl];
+            };
+            auto zpv = [&](int m) -> float {
+                const int sl = dz_slot(m);
+                return (zp_self || sl < 0) ? A(m, acell) : dz_fwd[sl];
+            };
+            const int pos[3] = {qx0, qy0, pb};
+            float qc[5], dQ[5][3];
+#pragma unroll
+            for (int m = 0; m < 5; ++m) {
+                qc[m] = A(m, acell);
+                const Deriv<float> d = q_deriv(bc, m, pos, qc[m], A(m, lxm), A(m, lxp), A(m, lym), A(m, lyp),
+                                               zmv(m), zpv(m));
+                dQ[m][0] = d.d[0];
+                dQ[m][1] = d.d[1];
+                dQ[m][2] = d.d[2];
+            }
+            float dS[8][3];
+#pragma unroll
+            for (int m = 0; m < 8; ++m) {
+                dS[m][0] = (A(5 + m, lxp) - A(5 + m, lxm)) / 2.0f;
+                dS[m][1] = (A(5 + m, lyp) - A(5 + m, lym)) / 2.0f;
+                dS[m][2] = (zpv(5 + m) - zmv(5 + m)) / 2.0f;
+            }
+            const float uo[3] = {hin[0][5], hin[0][6], hin[0][7]};   // u^{n-1} at (gx, gy, pb)
+            float F[3];
+            if (k.force_full) {
+                body_force(k, dQ, dS, uo, fe_prev, F);
+            } else {
+                F[0] = fe_prev[0];
+                F[1] = fe_prev[1];
+                F[2] = fe_prev[2];
+            }
+            const unsigned c = (unsigned)pb * plane + (unsigned)gy * nx + gx;
+            float pop[kQ];
+            gather32<PERIODIC>(f, bc, gx, gy, pb, c, pull, pop);
+            float rho, u[3];
+            collide_shifted(k, pop, F, rho, u);
+            const unsigned vs4 = (unsigned)f.vs * 4u, c4 = c * 4u;
+#pragma unroll
+            for (int j = 0; j < kQ; ++j) sto(f.gn, j * vs4 + c4, pop[j]);
+#pragma unroll
+            for (int m = 0; m < 3; ++m) sto(f.un[m], c4, u[m]);
+#pragma unroll
+            for (int m = 0; m < 5; ++m) sto(f.qn[m], c4, qc[m]);
+            if (aux) {
+                sto(f.rho, c4, rho);
+                sto(f.F[0], c4, F[0]);
+                sto(f.F[1], c4, F[1]);
+                sto(f.F[2], c4, F[2]);
+            }
+        }
+        // Plane pb becomes the lower z neighbour of the next plane.
+#pragma unroll
+        for (int m = 0; m < kNumA; ++m)
+            if (dz_slot(m) >= 0) dz_back[dz_slot(m)] = A(m, acell);
+    }
+}
+
+// ---- tile catalogue -----------------------------------------------------------
+struct TileShape {
+    int tx, ty, zc;
+    const char* name;
+};
+constexpr TileShape kTiles[] = {
+    {32, 8, 32, "32x8x32"}, {32, 8, 64, "32x8x64"}, {32, 8, 16, "32x8x16"},
+    {64, 4, 32, "64x4x32"}, {16, 16, 32, "16x16x32"}, {16, 16, 64, "16x16x64"},
+};
+constexpr int kNumTiles = sizeof(kTiles) / sizeof(kTiles[0]);
+
+int fused_tile_count() { return kNumTiles; }
+const char* fused_tile_name(int tile) { return (tile >= 0 && tile < kNumTiles) ? kTiles[tile].name : "?"; }
+
+template <int TX, int TY, int ZC>
+static void launch_tile(const Fields<float>& f, const BCDesc<float>& bc, const Phys<float>& k, int pull, int aux,
+                        hipStream_t s) {
+    const int ntx = (bc.n[0] + TX - 1) / TX, nty = (bc.n[1] + TY - 1) / TY, nzc = (bc.n[2] + ZC - 1) / ZC;
+    const int nblk = ntx * nty * nzc;
+    const int grid = (nblk + 7) / 8 * 8;
+    bool periodic = true;
+    for (int face = 0; face < 6; ++face)
+        periodic = periodic && bc.qkind[face] == QB_PERIODIC && bc.ukind[face] == UB_PERIODIC;
+    if (periodic)
+        hipLaunchKernelGGL((fused_step_kernel<TX, TY, ZC, true>), dim3(grid), dim3(TX * TY), 0, s, f, bc, k, pull,
+                           aux, ntx, nty, nzc);
+    else
+        hipLaunchKernelGGL((fused_step_kernel<TX, TY, ZC, false>), dim3(grid), dim3(TX * TY), 0, s, f, bc, k, pull,
+                           aux, ntx, nty, nzc);
+}
+
+void launch_fused_step_f32(const Fields<float>& f, const BCDesc<float>& bc, const Phys<float>& k, int tile,
+                           int pull, int aux, hipStream_t s) {
+    switch (tile) {
+        case 0: launch_tile<32, 8, 32>(f, bc, k, pull, aux, s); break;
+        case 1: launch_tile<32, 8, 64>(f, bc, k, pull, aux, s); break;
+        case 2: launch_tile<32, 8, 16>(f, bc, k, pull, aux, s); break;
+        case 3: launch_tile<64, 4, 32>(f, bc, k, pull, aux, s); break;
+        case 4: launch_tile<16, 16, 32>(f, bc, k, pull, aux, s); break;
+        default: launch_tile<16, 16, 64>(f, bc, k, pull, aux, s); break;
+    }
+}
+
 }  // namespace lbm

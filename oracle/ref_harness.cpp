@@ -1,9 +1,9 @@
 // ref_harness.cpp — drives the reference's own CPU solver core (compiled from
-// its sources under /root/reference/src by oracle/Makefile) on a state read
+// its sources under /root/reference/src by oracle/build_ref.sh) on a state read
 // from a file, and writes the resulting state. TEST INFRASTRUCTURE ONLY: its
 // outputs pin the oracle (oracle/lbm_oracle.cpp) and become the committed
 // fixtures under tests/golden/. Physical constants come from the params.h
-// that the Makefile puts first on the include path, exactly as the
+// that oracle/build_ref.sh puts first on the include path, exactly as the
 // reference's tests shadow it (tests/CMakeLists.txt:3-9).
 //
 // usage: ref_<scenario> <bc> <mode> <force_model> <steps> <in.bin> <out.bin>
