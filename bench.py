@@ -152,7 +152,12 @@ def main():
     step_frac = STEP_BYTES["fp32"] * V / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS
     log(f"FP32: {ms_step:.4f} ms/step, {mlups:.1f} MLUPS, kernels {prof}")
 
+    from lbman3d_amd.solver import bandwidth_probe
+    copy_gbs, copy_ms = bandwidth_probe(1 << 30, 20)
+    log(f"copy kernel: {copy_gbs:.0f} GB/s")
     secondary = {"fp32_step_hbm_frac": round(step_frac, 4), "fp32_kernels_ms": prof,
+                 "hbm_copy_GBs": round(copy_gbs, 1),
+                 "fp32_frac_of_copy_bw": round(STEP_BYTES["fp32"] * V / (ms_step * 1e-3) / 1e9 / copy_gbs, 4),
                  "fp32_wall_ms_per_step": round(wall * 1e3 / args.steps, 4),
                  "fp32_umax": diag["umax"], "fp32_nonfinite": diag["nonfinite"]}
     if tuned:

@@ -195,6 +195,10 @@ int lbm_tile_name(int32_t schedule, char* buf, size_t len);
  * keep the fastest; results per schedule in ms_out[count] (ms per step). */
 int lbm_autotune(lbm_handle h, int32_t n, double* ms_out, int32_t max, int32_t* count);
 
+/* HBM calibration: device-to-device copy of `bytes` (16-byte accesses),
+ * repeated `reps` times; returns read+write GB/s and ms per copy. */
+int lbm_bandwidth_probe(size_t bytes, int32_t reps, double* gbs, double* ms_per_copy);
+
 /* ---- host helpers -------------------------------------------------------- */
 /* Seeded random-director initial Q: per cell a director uniform on the unit
  * sphere and S = s0 + U(-noise, noise), Q = S (n n - I/3), from a

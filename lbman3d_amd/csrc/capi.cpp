@@ -938,6 +938,38 @@ int lbm_autotune(lbm_handle s, int32_t n, double* ms_out, int32_t max, int32_t* 
     return LBM_OK;
 }
 
+int lbm_bandwidth_probe(size_t bytes, int32_t reps, double* gbs, double* ms_per_copy) {
+    if (!gbs || reps < 1 || bytes < 16) return fail(LBM_ERR_INVALID, "bad argument");
+    bytes = bytes / 16 * 16;
+    void *a = nullptr, *b = nullptr;
+    HIPCHK(hipMalloc(&a, bytes));
+    if (hipMalloc(&b, bytes) != hipSuccess) {
+        (void)hipFree(a);
+        return fail(LBM_ERR_HIP, "hipMalloc failed");
+    }
+    hipStream_t st;
+    HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipMemsetAsync(a, 0, bytes, st));
+    launch_copy(a, b, bytes, st);  // warm-up
+    HIPCHK(hipEventRecord(e0, st));
+    for (int r = 0; r < reps; ++r) launch_copy(r & 1 ? b : a, r & 1 ? a : b, bytes, st);
+    HIPCHK(hipEventRecord(e1, st));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0.0f;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    *gbs = 2.0 * (double)bytes * reps / (ms * 1e-3) / 1e9;
+    if (ms_per_copy) *ms_per_copy = ms / reps;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipStreamDestroy(st);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    return LBM_OK;
+}
+
 int lbm_debug_slot_table(const lbm_config* cfg, uint16_t* table) {
     if (!cfg || !table) return fail(LBM_ERR_INVALID, "null argument");
     if (int r = validate_config(*cfg)) return r;

@@ -137,7 +137,7 @@ __device__ __forceinline__ void load_column(const Fields<float>& f, int qcol, in
 
 // Stage A at one column: Beris-Edwards update from the input plane in LDS
 // (xy neighbours) and the column's own z neighbours.
-template <int TX, int TY>
+template <int TX, int TY, bool NOGHOST>
 __device__ __forceinline__ void stage_a(const BCDesc<float>& bc, const Phys<float>& k,
                                         const float (*sIn)[TileGeom<TX, TY>::NI], int ii, int jj,
                                         const float zm[kNumIn], const float c[kNumIn], const float zp[kNumIn],
@@ -150,18 +150,137 @@ __device__ __forceinline__ void stage_a(const BCDesc<float>& bc, const Phys<floa
 #pragma unroll
     for (int m = 0; m < 5; ++m) {
         Q[m] = c[m];
-        dq[m] = q_deriv(bc, m, qpos, c[m], sIn[m][xm], sIn[m][xp], sIn[m][ym], sIn[m][yp], zm[m], zp[m]);
+        dq[m] = q_deriv<float, NOGHOST>(bc, m, qpos, c[m], sIn[m][xm], sIn[m][xp], sIn[m][ym], sIn[m][yp], zm[m], zp[m]);
     }
     float u[3], g[3][3];
 #pragma unroll
     for (int A = 0; A < 3; ++A) {
         const float v0 = c[5 + A];
         u[A] = v0;
-        g[A][0] = u_grad(bc, 0, A, upos[0], sIn[5 + A][xm], v0, sIn[5 + A][xp]);
-        g[A][1] = u_grad(bc, 1, A, upos[1], sIn[5 + A][ym], v0, sIn[5 + A][yp]);
-        g[A][2] = u_grad(bc, 2, A, upos[2], zm[5 + A], v0, zp[5 + A]);
+        g[A][0] = u_grad<float, NOGHOST>(bc, 0, A, upos[0], sIn[5 + A][xm], v0, sIn[5 + A][xp]);
+        g[A][1] = u_grad<float, NOGHOST>(bc, 1, A, upos[1], sIn[5 + A][ym], v0, sIn[5 + A][yp]);
+        g[A][2] = u_grad<float, NOGHOST>(bc, 2, A, upos[2], zm[5 + A], v0, zp[5 + A]);
     }
     q_pointwise(k, Q, u, dq, g, out.qn, out.sg, out.ta, fe);
+}
+
+// Register-lean form of the Q update for the FP32 fused kernels: the same
+// formulas as q_pointwise (src/model.h:102-342), evaluated component by
+// component so that the 35 stencil derivatives are never live together.
+// Operand order inside each formula follows q_pointwise; the Ericksen force
+// is accumulated per component (a different summation order, within FP32
+// rounding of q_pointwise).
+template <int TX, int TY, bool NOGHOST>
+__device__ __forceinline__ void stage_a_lean(const BCDesc<float>& bc, const Phys<float>& k,
+                                             const float (*sIn)[TileGeom<TX, TY>::NI], int ii, int jj,
+                                             const float zm[kNumIn], const float c[kNumIn],
+                                             const float zp[kNumIn], const int qpos[3], const int upos[3],
+                                             StageA& out, float fe[3]) {
+    using G = TileGeom<TX, TY>;
+    const int cc = jj * G::IX + ii;
+    const int xm = cc - 1, xp = cc + 1, ym = cc - G::IX, yp = cc + G::IX;
+    // Velocity gradient -> strain rate E and vorticity W.
+    float g[3][3];
+#pragma unroll
+    for (int A = 0; A < 3; ++A) {
+        const float v0 = c[5 + A];
+        g[A][0] = u_grad<float, NOGHOST>(bc, 0, A, upos[0], sIn[5 + A][xm], v0, sIn[5 + A][xp]);
+        g[A][1] = u_grad<float, NOGHOST>(bc, 1, A, upos[1], sIn[5 + A][ym], v0, sIn[5 + A][yp]);
+        g[A][2] = u_grad<float, NOGHOST>(bc, 2, A, upos[2], zm[5 + A], v0, zp[5 + A]);
+    }
+    const float Exx = g[0][0], Eyy = g[1][1];
+    const float Exy = 0.5f * (g[0][1] + g[1][0]);
+    const float Exz = 0.5f * (g[0][2] + g[2][0]);
+    const float Eyz = 0.5f * (g[1][2] + g[2][1]);
+    const float Wxy = 0.5f * (g[0][1] - g[1][0]);
+    const float Wxz = 0.5f * (g[0][2] - g[2][0]);
+    const float Wyz = 0.5f * (g[1][2] - g[2][1]);
+    const float Qxx = c[0], Qxy = c[1], Qxz = c[2], Qyy = c[3], Qyz = c[4];
+    const float u0 = c[5], u1 = c[6], u2 = c[7];
+    const float TrQ2 = 2.0f * (Qxx * Qxx + Qyy * Qyy + Qxx * Qyy + Qxy * Qxy + Qxz * Qxz + Qyz * Qyz);
+    const float third = 1.0f / 3.0f;
+    const float ld = k.A + k.C * TrQ2;
+
+    float H[5], adv[5];
+    float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+    float dxx0 = 0.0f, dxx1 = 0.0f, dxx2 = 0.0f;   // grad Qxx, kept for the cross term with Hyy
+#pragma unroll
+    for (int m = 0; m < 5; ++m) {
+        const Deriv<float> d = q_deriv<float, NOGHOST>(bc, m, qpos, c[m], sIn[m][xm], sIn[m][xp], sIn[m][ym], sIn[m][yp], zm[m], zp[m]);
+        float Q2;
+        switch (m) {
+            case 0: Q2 = Qxx * Qxx + Qxy * Qxy + Qxz * Qxz - third * TrQ2; break;
+            case 1: Q2 = Qxx * Qxy + Qxy * Qyy + Qxz * Qyz; break;
+            case 2: Q2 = Qxy * Qyz - Qxz * Qyy; break;
+            case 3: Q2 = Qxy * Qxy + Qyy * Qyy + Qyz * Qyz - third * TrQ2; break;
+            default: Q2 = Qxy * Qxz - Qyz * Qxx; break;
+        }
+        H[m] = k.L * d.lap - ld * c[m] - k.B * Q2;
+        adv[m] = -(adv_axis(k.advection, u0, d.d[0], d.d2[0]) + adv_axis(k.advection, u1, d.d[1], d.d2[1])
+                   + adv_axis(k.advection, u2, d.d[2], d.d2[2]));
+        // -H : grad_a Q over the nine index pairs (Qzz = -Qxx - Qyy, Hzz = -Hxx - Hyy).
+        fx -= 2.0f * H[m] * d.d[0];
+        fy -= 2.0f * H[m] * d.d[1];
+        fz -= 2.0f * H[m] * d.d[2];
+        if (m == 0) { dxx0 = d.d[0]; dxx1 = d.d[1]; dxx2 = d.d[2]; }
+        if (m == 3) {
+            fx -= H[0] * d.d[0] + H[3] * dxx0;
+            fy -= H[0] * d.d[1] + H[3] * dxx1;
+            fz -= H[0] * d.d[2] + H[3] * dxx2;
+        }
+    }
+    fe[0] = fx;
+    fe[1] = fy;
+    fe[2] = fz;
+    const float Hxx = H[0], Hxy = H[1], Hxz = H[2], Hyy = H[3], Hyz = H[4];
+
+    const float two3 = 2.0f / 3.0f;
+    const float trQH = (2.0f * Hxx + Hyy) * Qxx + (2.0f * Hxy) * Qxy + (2.0f * Hxz) * Qxz + (Hxx + 2.0f * Hyy) * Qyy
+                     + 2.0f * Hyz * Qyz;
+    out.ta[0] = (Hxy * Qxx + Hyy * Qxy + Hyz * Qxz) - (Qxy * Hxx + Qyy * Hxy + Qyz * Hxz);
+    out.ta[1] = (Hxz * Qxx + Hyz * Qxy + (-Hxx - Hyy) * Qxz) - (Qxz * Hxx + Qyz * Hxy + (-Qxx - Qyy) * Hxz);
+    out.ta[2] = (Hxz * Qxy + Hyz * Qyy + (-Hxx - Hyy) * Qyz) - (Qxz * Hxy + Qyz * Hyy + (-Qxx - Qyy) * Hyz);
+    const float trQE = (2.0f * Exx + Eyy) * Qxx + (2.0f * Exy) * Qxy + (2.0f * Exz) * Qxz + (Exx + 2.0f * Eyy) * Qyy
+                     + 2.0f * Eyz * Qyz;
+#pragma unroll
+    for (int m = 0; m < 5; ++m) {
+        float QH, aln2, cor, E;
+        switch (m) {
+            case 0:
+                QH = 2.0f * (Hxx * Qxx + Hxy * Qxy + Hxz * Qxz) - two3 * trQH;
+                aln2 = 2.0f * (Exx * Qxx + Exy * Qxy + Exz * Qxz) - two3 * trQE;
+                cor = 2.0f * (Qxy * Wxy + Qxz * Wxz);
+                E = Exx;
+                break;
+            case 1:
+                QH = Hxy * Qxx + Hyy * Qxy + Hyz * Qxz + Qxy * Hxx + Qyy * Hxy + Qyz * Hxz;
+                aln2 = Exy * Qxx + Eyy * Qxy + Eyz * Qxz + Qxy * Exx + Qyy * Exy + Qyz * Exz;
+                cor = -Qxx * Wxy + Qxz * Wyz + Qyy * Wxy + Qyz * Wxz;
+                E = Exy;
+                break;
+            case 2:
+                QH = Hxz * Qxx + Hyz * Qxy + (-Hxx - Hyy) * Qxz + Qxz * Hxx + Qyz * Hxy + (-Qxx - Qyy) * Hxz;
+                aln2 = Exz * Qxx + Eyz * Qxy + (-Exx - Eyy) * Qxz + Qxz * Exx + Qyz * Exy + (-Qxx - Qyy) * Exz;
+                cor = -2.0f * Qxx * Wxz - Qxy * Wyz - Qyy * Wxz + Qyz * Wxy;
+                E = Exz;
+                break;
+            case 3:
+                QH = 2.0f * (Hxy * Qxy + Hyy * Qyy + Hyz * Qyz) - two3 * trQH;
+                aln2 = 2.0f * (Exy * Qxy + Eyy * Qyy + Eyz * Qyz) - two3 * trQE;
+                cor = 2.0f * (-Qxy * Wxy + Qyz * Wyz);
+                E = Eyy;
+                break;
+            default:
+                QH = Hxz * Qxy + Hyz * Qyy + (-Hxx - Hyy) * Qyz + Qxz * Hxy + Qyz * Hyy + (-Qxx - Qyy) * Hyz;
+                aln2 = Exz * Qxy + Eyz * Qyy + (-Exx - Eyy) * Qyz + Qxz * Exy + Qyz * Eyy + (-Qxx - Qyy) * Eyz;
+                cor = -Qxx * Wyz - Qxy * Wxz - Qxz * Wxy - 2.0f * Qyy * Wyz;
+                E = Eyz;
+                break;
+        }
+        const float aln3 = -2.0f * trQE * c[m];
+        out.sg[m] = -two3 * k.lambda * H[m] - k.lambda * QH + 2.0f * k.lambda * trQH * c[m];
+        out.qn[m] = c[m] + k.dt * (adv[m] + cor + k.lambda * (two3 * E + aln2 + aln3) + k.gamma * H[m]);
+    }
 }
 
 // Components of the stage-A results whose z differences the force needs:
@@ -345,7 +464,7 @@ fused_step_kernel(Fields<float> f, BCDesc<float> bc, Phys<float> k, int pull, in
             upos[2] = zu;
             StageA a;
             float fe[3];
-            stage_a<TX, TY>(bc, k, sIn, ci + 1, cj + 1, zm, c, zp, qpos, upos, a, fe);
+            stage_a<TX, TY, PERIODIC>(bc, k, sIn, ci + 1, cj + 1, zm, c, zp, qpos, upos, a, fe);
 #pragma unroll
             for (int m = 0; m < kNumA; ++m) sA[par][m][acol] = stage_value(a, m);
             if (slot == 0) {
@@ -385,7 +504,7 @@ fused_step_kernel(Fields<float> f, BCDesc<float> bc, Phys<float> k, int pull, in
 #pragma unroll
             for (int m = 0; m < 5; ++m) {
                 qc[m] = A(m, acell);
-                const Deriv<float> d = q_deriv(bc, m, pos, qc[m], A(m, lxm), A(m, lxp), A(m, lym), A(m, lyp),
+                const Deriv<float> d = q_deriv<float, PERIODIC>(bc, m, pos, qc[m], A(m, lxm), A(m, lxp), A(m, lym), A(m, lyp),
                                                zmv(m), zpv(m));
                 dQ[m][0] = d.d[0];
                 dQ[m][1] = d.d[1];
@@ -433,21 +552,246 @@ fused_step_kernel(Fields<float> f, BCDesc<float> bc, Phys<float> k, int pull, in
     }
 }
 
+// Second pipeline: the same computation with one stage-A plane in LDS (the
+// force stage reads plane pb while the fresh plane pa waits in registers and
+// is published after a barrier), and with every global load issued one z
+// plane ahead of its use: the pull gather of plane pb and the input planes
+// pa + 2 are in flight while the Q update of plane pa computes.
+template <int TX, int TY, int ZC, bool PERIODIC, int WPE>
+__global__ void __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(WPE, WPE)))
+fused_step_kernel_pf(Fields<float> f, BCDesc<float> bc, Phys<float> k, int pull, int aux, int ntx, int nty,
+                     int nzc) {
+    using G = TileGeom<TX, TY>;
+    __shared__ float sIn[kNumIn][G::NI];           // input plane pa, extended region
+    __shared__ float sA[kNumA][G::NA];             // stage-A results of plane pa - 1
+    __shared__ float sRing[4][kNumIn][G::NR];      // ring-column inputs, planes mod 4
+
+    const int nblk = ntx * nty * nzc;
+    const int per_xcd = (nblk + 7) / 8;
+    const int L = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
+    if (L >= nblk) return;
+    const int tx = L % ntx, ty = (L / ntx) % nty, tz = L / (ntx * nty);
+    const int nx = bc.n[0], ny = bc.n[1], nz = bc.n[2];
+    const int x0 = tx * TX, y0 = ty * TY;
+    const int z0 = tz * ZC, z1 = min(z0 + ZC, nz);
+    const unsigned plane = (unsigned)nx * ny;
+
+    AxisTags ax[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        ax[a].qlo = PERIODIC || bc.qkind[2 * a] == QB_PERIODIC;
+        ax[a].qhi = PERIODIC || bc.qkind[2 * a + 1] == QB_PERIODIC;
+        ax[a].ulo = PERIODIC || bc.ukind[2 * a] == UB_PERIODIC;
+        ax[a].uhi = PERIODIC || bc.ukind[2 * a + 1] == UB_PERIODIC;
+    }
+
+    const int t = threadIdx.x;
+    const int ia = t % TX + 1, ib = t / TX + 1;
+    const int acell = ib * G::AX + ia;
+    const int gx = x0 + ia - 1, gy = y0 + ib - 1;
+    const bool active = gx < nx && gy < ny;
+    const bool has_ring = t < G::NR;
+    int ra = 0, rb = 0;
+    if (has_ring) ring_col(t, G::AX, G::AY, ra, rb);
+    const int orank = t - (G::NT - G::NO);
+    const bool has_outer = orank >= 0;
+    int oi = 0, oj = 0;
+    if (has_outer) ring_col(orank, G::IX, G::IY, oi, oj);
+
+    const int qx0 = map_coord(gx, nx, ax[0].qlo, ax[0].qhi), qy0 = map_coord(gy, ny, ax[1].qlo, ax[1].qhi);
+    const int ux0 = map_coord(gx, nx, ax[0].ulo, ax[0].uhi), uy0 = map_coord(gy, ny, ax[1].ulo, ax[1].uhi);
+    const int rgx = x0 - 1 + ra, rgy = y0 - 1 + rb;
+    const int qxr = map_coord(rgx, nx, ax[0].qlo, ax[0].qhi), qyr = map_coord(rgy, ny, ax[1].qlo, ax[1].qhi);
+    const int uxr = map_coord(rgx, nx, ax[0].ulo, ax[0].uhi), uyr = map_coord(rgy, ny, ax[1].ulo, ax[1].uhi);
+    int oqc = 0, ouc = 0;
+    if (has_outer) {
+        const int ogx = x0 - 2 + oi, ogy = y0 - 2 + oj;
+        oqc = map_coord(ogy, ny, ax[1].qlo, ax[1].qhi) * nx + map_coord(ogx, nx, ax[0].qlo, ax[0].qhi);
+        ouc = map_coord(ogy, ny, ax[1].ulo, ax[1].uhi) * nx + map_coord(ogx, nx, ax[0].ulo, ax[0].uhi);
+    }
+    const int qc0 = qy0 * nx + qx0, uc0 = uy0 * nx + ux0;
+    const int qcr = qyr * nx + qxr, ucr = uyr * nx + uxr;
+    auto qplane = [&](int z) { return (unsigned)map_coord(z, nz, ax[2].qlo, ax[2].qhi) * plane; };
+    auto uplane = [&](int z) { return (unsigned)map_coord(z, nz, ax[2].ulo, ax[2].uhi) * plane; };
+    const int cc_int = (ib + 1) * G::IX + ia + 1;
+    const int cc_ring = (rb + 1) * G::IX + ra + 1;
+    const int cc_out = oj * G::IX + oi;
+    const int rcell = rb * G::AX + ra;
+
+    // Prologue: interior planes z0-2 .. z0, ring planes z0-2 .. z0, outer plane z0-1.
+    float hin[3][kNumIn];
+    load_column<TX, TY>(f, qc0, uc0, qplane(z0 - 2), uplane(z0 - 2), hin[0]);
+    load_column<TX, TY>(f, qc0, uc0, qplane(z0 - 1), uplane(z0 - 1), hin[1]);
+    load_column<TX, TY>(f, qc0, uc0, qplane(z0), uplane(z0), hin[2]);
+    if (has_ring) {
+#pragma unroll 1
+        for (int z = z0 - 2; z <= z0; ++z) {
+            float v[kNumIn];
+            load_column<TX, TY>(f, qcr, ucr, qplane(z), uplane(z), v);
+#pragma unroll
+            for (int m = 0; m < kNumIn; ++m) sRing[z & 3][m][t] = v[m];
+        }
+    }
+    float ov[kNumIn];
+    if (has_outer) load_column<TX, TY>(f, oqc, ouc, qplane(z0 - 1), uplane(z0 - 1), ov);
+
+    float dz_back[kNumDZ];
+#pragma unroll
+    for (int m = 0; m < kNumDZ; ++m) dz_back[m] = 0.0f;
+    float fe_prev[3] = {0, 0, 0}, fe_cur[3] = {0, 0, 0};
+
+    for (int pa = z0 - 1; pa <= z1; ++pa) {
+        const int pb = pa - 1;
+        const bool do_b = pb >= z0 && active;
+        // ---- step 1: issue next-plane loads and the pull gather; stage plane pa ----
+        float hnext[kNumIn], rnext[kNumIn], onext[kNumIn], pop[kQ];
+        load_column<TX, TY>(f, qc0, uc0, qplane(pa + 2), uplane(pa + 2), hnext);
+        if (has_ring) load_column<TX, TY>(f, qcr, ucr, qplane(pa + 2), uplane(pa + 2), rnext);
+        if (has_outer) load_column<TX, TY>(f, oqc, ouc, qplane(pa + 1), uplane(pa + 1), onext);
+        const unsigned cb = (unsigned)pb * plane + (unsigned)gy * nx + gx;
+        if (do_b) gather32<PERIODIC>(f, bc, gx, gy, pb, cb, pull, pop);
+#pragma unroll
+        for (int m = 0; m < kNumIn; ++m) sIn[m][cc_int] = hin[1][m];
+        if (has_ring) {
+#pragma unroll
+            for (int m = 0; m < kNumIn; ++m) sIn[m][cc_ring] = sRing[pa & 3][m][t];
+        }
+        if (has_outer) {
+#pragma unroll
+            for (int m = 0; m < kNumIn; ++m) sIn[m][cc_out] = ov[m];
+        }
+        __syncthreads();
+
+        // ---- step 2: Q update at plane pa (results stay in registers) ----
+        const int zq = map_coord(pa, nz, ax[2].qlo, ax[2].qhi), zu = map_coord(pa, nz, ax[2].ulo, ax[2].uhi);
+        fe_prev[0] = fe_cur[0];
+        fe_prev[1] = fe_cur[1];
+        fe_prev[2] = fe_cur[2];
+        StageA ai, ar;
+        {
+            const int qpos[3] = {qx0, qy0, zq};
+            const int upos[3] = {ux0, uy0, zu};
+            stage_a_lean<TX, TY, PERIODIC>(bc, k, sIn, ia + 1, ib + 1, hin[0], hin[1], hin[2], qpos, upos, ai, fe_cur);
+        }
+        if (has_ring) {
+            const int qpos[3] = {qxr, qyr, zq};
+            const int upos[3] = {uxr, uyr, zu};
+            float zm[kNumIn], c[kNumIn], zp[kNumIn], fer[3];
+#pragma unroll
+            for (int m = 0; m < kNumIn; ++m) {
+                zm[m] = sRing[(pa - 1) & 3][m][t];
+                c[m] = sRing[pa & 3][m][t];
+                zp[m] = sRing[(pa + 1) & 3][m][t];
+            }
+            stage_a_lean<TX, TY, PERIODIC>(bc, k, sIn, ra + 1, rb + 1, zm, c, zp, qpos, upos, ar, fer);
+        }
+
+        // ---- step 3: force + collide-stream at plane pb (sA holds plane pb) ----
+        if (do_b) {
+            const int lxm = (qx0 == 0 && !ax[0].qlo) ? acell : acell - 1;
+            const int lxp = (qx0 == nx - 1 && !ax[0].qhi) ? acell : acell + 1;
+            const int lym = (qy0 == 0 && !ax[1].qlo) ? acell : acell - G::AX;
+            const int lyp = (qy0 == ny - 1 && !ax[1].qhi) ? acell : acell + G::AX;
+            const bool zm_self = (pb == 0 && !ax[2].qlo);
+            const bool zp_self = (pb == nz - 1 && !ax[2].qhi);
+            auto zmv = [&](int m) -> float {
+                const int sl = dz_slot(m);
+                return (zm_self || sl < 0) ? sA[m][acell] : dz_back[sl];
+            };
+            auto zpv = [&](int m) -> float {
+                const int sl = dz_slot(m);
+                return (zp_self || sl < 0) ? sA[m][acell] : stage_value(ai, m);
+            };
+            const int pos[3] = {qx0, qy0, pb};
+            float qc[5], dQ[5][3];
+#pragma unroll
+            for (int m = 0; m < 5; ++m) {
+                qc[m] = sA[m][acell];
+                const Deriv<float> d = q_deriv<float, PERIODIC>(bc, m, pos, qc[m], sA[m][lxm], sA[m][lxp], sA[m][lym], sA[m][lyp],
+                                               zmv(m), zpv(m));
+                dQ[m][0] = d.d[0];
+                dQ[m][1] = d.d[1];
+                dQ[m][2] = d.d[2];
+            }
+            float dS[8][3];
+#pragma unroll
+            for (int m = 0; m < 8; ++m) {
+                dS[m][0] = (sA[5 + m][lxp] - sA[5 + m][lxm]) / 2.0f;
+                dS[m][1] = (sA[5 + m][lyp] - sA[5 + m][lym]) / 2.0f;
+                dS[m][2] = (zpv(5 + m) - zmv(5 + m)) / 2.0f;
+            }
+            const float uo[3] = {hin[0][5], hin[0][6], hin[0][7]};   // u^{n-1} at (gx, gy, pb)
+            float F[3];
+            if (k.force_full) {
+                body_force(k, dQ, dS, uo, fe_prev, F);
+            } else {
+                F[0] = fe_prev[0];
+                F[1] = fe_prev[1];
+                F[2] = fe_prev[2];
+            }
+            float rho, u[3];
+            collide_shifted(k, pop, F, rho, u);
+            const unsigned vs4 = (unsigned)f.vs * 4u, c4 = cb * 4u;
+#pragma unroll
+            for (int j = 0; j < kQ; ++j) sto(f.gn, j * vs4 + c4, pop[j]);
+#pragma unroll
+            for (int m = 0; m < 3; ++m) sto(f.un[m], c4, u[m]);
+#pragma unroll
+            for (int m = 0; m < 5; ++m) sto(f.qn[m], c4, qc[m]);
+            if (aux) {
+                sto(f.rho, c4, rho);
+                sto(f.F[0], c4, F[0]);
+                sto(f.F[1], c4, F[1]);
+                sto(f.F[2], c4, F[2]);
+            }
+        }
+        // Plane pb becomes the lower z neighbour of the next plane.
+#pragma unroll
+        for (int m = 0; m < kNumA; ++m)
+            if (dz_slot(m) >= 0) dz_back[dz_slot(m)] = sA[m][acell];
+        __syncthreads();
+
+        // ---- step 4: publish stage A of plane pa; advance histories ----
+#pragma unroll
+        for (int m = 0; m < kNumA; ++m) sA[m][acell] = stage_value(ai, m);
+        if (has_ring) {
+#pragma unroll
+            for (int m = 0; m < kNumA; ++m) sA[m][rcell] = stage_value(ar, m);
+#pragma unroll
+            for (int m = 0; m < kNumIn; ++m) sRing[(pa + 2) & 3][m][t] = rnext[m];
+        }
+#pragma unroll
+        for (int m = 0; m < kNumIn; ++m) {
+            hin[0][m] = hin[1][m];
+            hin[1][m] = hin[2][m];
+            hin[2][m] = hnext[m];
+        }
+        if (has_outer) {
+#pragma unroll
+            for (int m = 0; m < kNumIn; ++m) ov[m] = onext[m];
+        }
+    }
+}
+
 // ---- tile catalogue -----------------------------------------------------------
+// kind 0: first pipeline (stage-A planes double-buffered in LDS);
+// kind 1: prefetch pipeline (single stage-A plane, loads one plane ahead).
 struct TileShape {
-    int tx, ty, zc;
+    int kind, tx, ty, zc;
     const char* name;
 };
 constexpr TileShape kTiles[] = {
-    {32, 8, 32, "32x8x32"}, {32, 8, 64, "32x8x64"}, {32, 8, 16, "32x8x16"},
-    {64, 4, 32, "64x4x32"}, {16, 16, 32, "16x16x32"}, {16, 16, 64, "16x16x64"},
+    {0, 32, 8, 32, "32x8x32"},      {0, 32, 8, 64, "32x8x64"},       {0, 32, 8, 16, "32x8x16"},
+    {0, 64, 4, 32, "64x4x32"},      {0, 16, 16, 32, "16x16x32"},     {0, 16, 16, 64, "16x16x64"},
+    {1, 32, 8, 64, "pf:32x8x64"},   {1, 32, 8, 32, "pf:32x8x32"},    {1, 16, 16, 64, "pf:16x16x64"},
+    {1, 64, 4, 64, "pf:64x4x64"},
 };
 constexpr int kNumTiles = sizeof(kTiles) / sizeof(kTiles[0]);
 
 int fused_tile_count() { return kNumTiles; }
 const char* fused_tile_name(int tile) { return (tile >= 0 && tile < kNumTiles) ? kTiles[tile].name : "?"; }
 
-template <int TX, int TY, int ZC>
+template <int KIND, int TX, int TY, int ZC>
 static void launch_tile(const Fields<float>& f, const BCDesc<float>& bc, const Phys<float>& k, int pull, int aux,
                         hipStream_t s) {
     const int ntx = (bc.n[0] + TX - 1) / TX, nty = (bc.n[1] + TY - 1) / TY, nzc = (bc.n[2] + ZC - 1) / ZC;
@@ -456,24 +800,48 @@ static void launch_tile(const Fields<float>& f, const BCDesc<float>& bc, const P
     bool periodic = true;
     for (int face = 0; face < 6; ++face)
         periodic = periodic && bc.qkind[face] == QB_PERIODIC && bc.ukind[face] == UB_PERIODIC;
-    if (periodic)
-        hipLaunchKernelGGL((fused_step_kernel<TX, TY, ZC, true>), dim3(grid), dim3(TX * TY), 0, s, f, bc, k, pull,
-                           aux, ntx, nty, nzc);
-    else
-        hipLaunchKernelGGL((fused_step_kernel<TX, TY, ZC, false>), dim3(grid), dim3(TX * TY), 0, s, f, bc, k, pull,
-                           aux, ntx, nty, nzc);
+    if (KIND == 0) {
+        if (periodic)
+            hipLaunchKernelGGL((fused_step_kernel<TX, TY, ZC, true>), dim3(grid), dim3(TX * TY), 0, s, f, bc, k,
+                               pull, aux, ntx, nty, nzc);
+        else
+            hipLaunchKernelGGL((fused_step_kernel<TX, TY, ZC, false>), dim3(grid), dim3(TX * TY), 0, s, f, bc, k,
+                               pull, aux, ntx, nty, nzc);
+    } else {
+        if (periodic)
+            hipLaunchKernelGGL((fused_step_kernel_pf<TX, TY, ZC, true, 2>), dim3(grid), dim3(TX * TY), 0, s, f, bc,
+                               k, pull, aux, ntx, nty, nzc);
+        else
+            hipLaunchKernelGGL((fused_step_kernel_pf<TX, TY, ZC, false, 2>), dim3(grid), dim3(TX * TY), 0, s, f, bc,
+                               k, pull, aux, ntx, nty, nzc);
+    }
 }
 
 void launch_fused_step_f32(const Fields<float>& f, const BCDesc<float>& bc, const Phys<float>& k, int tile,
                            int pull, int aux, hipStream_t s) {
     switch (tile) {
-        case 0: launch_tile<32, 8, 32>(f, bc, k, pull, aux, s); break;
-        case 1: launch_tile<32, 8, 64>(f, bc, k, pull, aux, s); break;
-        case 2: launch_tile<32, 8, 16>(f, bc, k, pull, aux, s); break;
-        case 3: launch_tile<64, 4, 32>(f, bc, k, pull, aux, s); break;
-        case 4: launch_tile<16, 16, 32>(f, bc, k, pull, aux, s); break;
-        default: launch_tile<16, 16, 64>(f, bc, k, pull, aux, s); break;
+        case 0: launch_tile<0, 32, 8, 32>(f, bc, k, pull, aux, s); break;
+        case 1: launch_tile<0, 32, 8, 64>(f, bc, k, pull, aux, s); break;
+        case 2: launch_tile<0, 32, 8, 16>(f, bc, k, pull, aux, s); break;
+        case 3: launch_tile<0, 64, 4, 32>(f, bc, k, pull, aux, s); break;
+        case 4: launch_tile<0, 16, 16, 32>(f, bc, k, pull, aux, s); break;
+        case 5: launch_tile<0, 16, 16, 64>(f, bc, k, pull, aux, s); break;
+        case 6: launch_tile<1, 32, 8, 64>(f, bc, k, pull, aux, s); break;
+        case 7: launch_tile<1, 32, 8, 32>(f, bc, k, pull, aux, s); break;
+        case 8: launch_tile<1, 16, 16, 64>(f, bc, k, pull, aux, s); break;
+        default: launch_tile<1, 64, 4, 64>(f, bc, k, pull, aux, s); break;
     }
+}
+
+// ---- HBM calibration: plain 16-byte-per-lane copy ------------------------------
+__global__ void __launch_bounds__(256) copy_kernel(const float4* __restrict__ src, float4* __restrict__ dst,
+                                                   long n4) {
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += (long)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+void launch_copy(const void* src, void* dst, size_t bytes, hipStream_t s) {
+    const long n4 = (long)(bytes / 16);
+    hipLaunchKernelGGL(copy_kernel, dim3(256 * 8 * 4), dim3(256), 0, s, (const float4*)src, (float4*)dst, n4);
 }
 
 }  // namespace lbm

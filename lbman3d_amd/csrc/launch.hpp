@@ -43,4 +43,7 @@ const char* fused_tile_name(int tile);
 void launch_fused_step_f32(const Fields<float>& f, const BCDesc<float>& bc, const Phys<float>& k,
                            int tile, int pull, int store_aux, hipStream_t s);
 
+// Plain device-to-device copy kernel (HBM bandwidth calibration).
+void launch_copy(const void* src, void* dst, size_t bytes, hipStream_t s);
+
 }  // namespace lbm

@@ -44,13 +44,16 @@ __device__ __forceinline__ void q_pair(const BCDesc<T>& bc, int a, int c, int i,
 
 // QGradientAndLaplacian for one component from its seven stencil values
 // (src/boundary_handler.h:287-320).
-template <typename T>
+// NOGHOST: every face periodic, so the fetched neighbours are used as they are.
+template <typename T, bool NOGHOST = false>
 __device__ __forceinline__ Deriv<T> q_deriv(const BCDesc<T>& bc, int c, const int pos[3], T q0,
                                             T xm, T xp, T ym, T yp, T zm, T zp) {
-    T mx, px, my, py, mz, pz;
-    q_pair(bc, 0, c, pos[0], xm, q0, xp, mx, px);
-    q_pair(bc, 1, c, pos[1], ym, q0, yp, my, py);
-    q_pair(bc, 2, c, pos[2], zm, q0, zp, mz, pz);
+    T mx = xm, px = xp, my = ym, py = yp, mz = zm, pz = zp;
+    if (!NOGHOST) {
+        q_pair(bc, 0, c, pos[0], xm, q0, xp, mx, px);
+        q_pair(bc, 1, c, pos[1], ym, q0, yp, my, py);
+        q_pair(bc, 2, c, pos[2], zm, q0, zp, mz, pz);
+    }
     Deriv<T> r;
     r.d[0] = (px - mx) / T(2.0);
     r.d[1] = (py - my) / T(2.0);
@@ -71,10 +74,10 @@ __device__ __forceinline__ T u_ghost(const BCDesc<T>& bc, int face, int comp, bo
     if (k == UB_SPECULAR) return normal ? (T(2.0) * wv - vb) : vb;
     return T(2.0) * wv - vb;
 }
-template <typename T>
+template <typename T, bool NOGHOST = false>
 __device__ __forceinline__ T u_grad(const BCDesc<T>& bc, int a, int comp, int i, T vm, T v0, T vp) {
     T m = vm, p = vp;
-    if (bc.ukind[2 * a] != UB_PERIODIC) {
+    if (!NOGHOST && bc.ukind[2 * a] != UB_PERIODIC) {
         const bool normal = (comp == a);
         if (i == 0) m = u_ghost(bc, 2 * a, comp, normal, v0);
         if (i == bc.n[a] - 1) p = u_ghost(bc, 2 * a + 1, comp, normal, v0);

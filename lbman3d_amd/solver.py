@@ -51,6 +51,7 @@ SIGNATURES = {
     "lbm_autotune": (C.c_int, [C.c_void_p, C.c_int32, _D, C.c_int32, C.POINTER(C.c_int32)]),
     "lbm_seed_director_field": (C.c_int, [C.c_uint64, C.c_int32, C.c_int32, C.c_int32, C.c_double,
                                           C.c_double, _D]),
+    "lbm_bandwidth_probe": (C.c_int, [C.c_size_t, C.c_int32, _D, _D]),
     "lbm_debug_slot_table": (C.c_int, [C.POINTER(LbmConfig), C.POINTER(C.c_uint16)]),
     "lbm_batch_create": (C.c_int, [C.POINTER(LbmConfig), C.c_int32, C.POINTER(C.c_void_p)]),
     "lbm_batch_destroy": (C.c_int, [C.c_void_p]),
@@ -111,6 +112,13 @@ def seed_director_field(seed, nx, ny, nz, s0=0.5, noise=0.01) -> np.ndarray:
     q = np.empty(5 * nx * ny * nz)
     check(lib().lbm_seed_director_field(seed, nx, ny, nz, s0, noise, _p(q)))
     return q.reshape(5, -1)
+
+
+def bandwidth_probe(nbytes=1 << 30, reps=20):
+    """Achievable HBM bandwidth of a plain copy kernel: (GB/s, ms per copy)."""
+    g, ms = C.c_double(), C.c_double()
+    check(lib().lbm_bandwidth_probe(nbytes, reps, C.byref(g), C.byref(ms)))
+    return g.value, ms.value
 
 
 def slot_table(cfg: LbmConfig) -> np.ndarray:
