@@ -29,9 +29,19 @@ namespace lbm {
 
 // Map a coordinate of the extended tile into the domain: periodic side wraps,
 // any other side clamps (the rule of nb_index for arbitrary distance).
+// The extended tile reaches at most two cells past a face, so a periodic wrap
+// needs at most two additions or subtractions of n (also for n = 1).
 __device__ __forceinline__ int map_coord(int t, int n, bool lo_per, bool hi_per) {
-    if (t < 0) return lo_per ? ((t % n) + n) % n : 0;
-    if (t >= n) return hi_per ? t % n : n - 1;
+    if (t < 0) {
+        if (!lo_per) return 0;
+        t += n;
+        return t < 0 ? t + n : t;
+    }
+    if (t >= n) {
+        if (!hi_per) return n - 1;
+        t -= n;
+        return t >= n ? t - n : t;
+    }
     return t;
 }
 
@@ -42,6 +52,14 @@ __device__ __forceinline__ float ldo(const float* base, unsigned boff) {
 }
 __device__ __forceinline__ void sto(float* base, unsigned boff, float v) {
     *reinterpret_cast<float*>(reinterpret_cast<char*>(base) + boff) = v;
+}
+// Store with an optional non-temporal hint (streaming output read back only
+// next step, so it need not displace the halo rows held in L2).
+template <bool NT>
+__device__ __forceinline__ void sto_hint(float* base, unsigned boff, float v) {
+    float* p = reinterpret_cast<float*>(reinterpret_cast<char*>(base) + boff);
+    if (NT) __builtin_nontemporal_store(v, p);
+    else *p = v;
 }
 
 // Pull gather with 32-bit offsets (fused kernel; requires 15 * vs * 4 < 2^32).
@@ -170,7 +188,7 @@ __device__ __forceinline__ void stage_a(const BCDesc<float>& bc, const Phys<floa
 // Operand order inside each formula follows q_pointwise; the Ericksen force
 // is accumulated per component (a different summation order, within FP32
 // rounding of q_pointwise).
-template <int TX, int TY, bool NOGHOST>
+template <int TX, int TY, bool NOGHOST, bool UPWIND>
 __device__ __forceinline__ void stage_a_lean(const BCDesc<float>& bc, const Phys<float>& k,
                                              const float (*sIn)[TileGeom<TX, TY>::NI], int ii, int jj,
                                              const float zm[kNumIn], const float c[kNumIn],
@@ -216,8 +234,8 @@ __device__ __forceinline__ void stage_a_lean(const BCDesc<float>& bc, const Phys
             default: Q2 = Qxy * Qxz - Qyz * Qxx; break;
         }
         H[m] = k.L * d.lap - ld * c[m] - k.B * Q2;
-        adv[m] = -(adv_axis(k.advection, u0, d.d[0], d.d2[0]) + adv_axis(k.advection, u1, d.d[1], d.d2[1])
-                   + adv_axis(k.advection, u2, d.d[2], d.d2[2]));
+        adv[m] = -(adv_axis(UPWIND, u0, d.d[0], d.d2[0]) + adv_axis(UPWIND, u1, d.d[1], d.d2[1])
+                   + adv_axis(UPWIND, u2, d.d[2], d.d2[2]));
         // -H : grad_a Q over the nine index pairs (Qzz = -Qxx - Qyy, Hzz = -Hxx - Hyy).
         fx -= 2.0f * H[m] * d.d[0];
         fy -= 2.0f * H[m] * d.d[1];
@@ -557,7 +575,7 @@ fused_step_kernel(Fields<float> f, BCDesc<float> bc, Phys<float> k, int pull, in
 // is published after a barrier), and with every global load issued one z
 // plane ahead of its use: the pull gather of plane pb and the input planes
 // pa + 2 are in flight while the Q update of plane pa computes.
-template <int TX, int TY, int ZC, bool PERIODIC, int WPE>
+template <int TX, int TY, int ZC, bool PERIODIC, int WPE, bool UPWIND, bool NT>
 __global__ void __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(WPE, WPE)))
 fused_step_kernel_pf(Fields<float> f, BCDesc<float> bc, Phys<float> k, int pull, int aux, int ntx, int nty,
                      int nzc) {
@@ -671,7 +689,7 @@ fused_step_kernel_pf(Fields<float> f, BCDesc<float> bc, Phys<float> k, int pull,
         {
             const int qpos[3] = {qx0, qy0, zq};
             const int upos[3] = {ux0, uy0, zu};
-            stage_a_lean<TX, TY, PERIODIC>(bc, k, sIn, ia + 1, ib + 1, hin[0], hin[1], hin[2], qpos, upos, ai, fe_cur);
+            stage_a_lean<TX, TY, PERIODIC, UPWIND>(bc, k, sIn, ia + 1, ib + 1, hin[0], hin[1], hin[2], qpos, upos, ai, fe_cur);
         }
         if (has_ring) {
             const int qpos[3] = {qxr, qyr, zq};
@@ -683,7 +701,7 @@ fused_step_kernel_pf(Fields<float> f, BCDesc<float> bc, Phys<float> k, int pull,
                 c[m] = sRing[pa & 3][m][t];
                 zp[m] = sRing[(pa + 1) & 3][m][t];
             }
-            stage_a_lean<TX, TY, PERIODIC>(bc, k, sIn, ra + 1, rb + 1, zm, c, zp, qpos, upos, ar, fer);
+            stage_a_lean<TX, TY, PERIODIC, UPWIND>(bc, k, sIn, ra + 1, rb + 1, zm, c, zp, qpos, upos, ar, fer);
         }
 
         // ---- step 3: force + collide-stream at plane pb (sA holds plane pb) ----
@@ -733,11 +751,11 @@ fused_step_kernel_pf(Fields<float> f, BCDesc<float> bc, Phys<float> k, int pull,
             collide_shifted(k, pop, F, rho, u);
             const unsigned vs4 = (unsigned)f.vs * 4u, c4 = cb * 4u;
 #pragma unroll
-            for (int j = 0; j < kQ; ++j) sto(f.gn, j * vs4 + c4, pop[j]);
+            for (int j = 0; j < kQ; ++j) sto_hint<NT>(f.gn, j * vs4 + c4, pop[j]);
 #pragma unroll
-            for (int m = 0; m < 3; ++m) sto(f.un[m], c4, u[m]);
+            for (int m = 0; m < 3; ++m) sto_hint<NT>(f.un[m], c4, u[m]);
 #pragma unroll
-            for (int m = 0; m < 5; ++m) sto(f.qn[m], c4, qc[m]);
+            for (int m = 0; m < 5; ++m) sto_hint<NT>(f.qn[m], c4, qc[m]);
             if (aux) {
                 sto(f.rho, c4, rho);
                 sto(f.F[0], c4, F[0]);
@@ -775,16 +793,16 @@ fused_step_kernel_pf(Fields<float> f, BCDesc<float> bc, Phys<float> k, int pull,
 
 // ---- tile catalogue -----------------------------------------------------------
 // kind 0: first pipeline (stage-A planes double-buffered in LDS);
-// kind 1: prefetch pipeline (single stage-A plane, loads one plane ahead).
+// kind 1: prefetch pipeline (single stage-A plane, loads one plane ahead);
+// kind 2: prefetch pipeline with non-temporal output stores.
 struct TileShape {
     int kind, tx, ty, zc;
     const char* name;
 };
 constexpr TileShape kTiles[] = {
-    {0, 32, 8, 32, "32x8x32"},      {0, 32, 8, 64, "32x8x64"},       {0, 32, 8, 16, "32x8x16"},
-    {0, 64, 4, 32, "64x4x32"},      {0, 16, 16, 32, "16x16x32"},     {0, 16, 16, 64, "16x16x64"},
-    {1, 32, 8, 64, "pf:32x8x64"},   {1, 32, 8, 32, "pf:32x8x32"},    {1, 16, 16, 64, "pf:16x16x64"},
-    {1, 64, 4, 64, "pf:64x4x64"},
+    {0, 32, 8, 32, "32x8x32"},        {0, 32, 8, 64, "32x8x64"},        {0, 64, 4, 32, "64x4x32"},
+    {1, 64, 4, 64, "pf:64x4x64"},     {1, 32, 8, 64, "pf:32x8x64"},     {1, 64, 4, 32, "pf:64x4x32"},
+    {1, 32, 8, 32, "pf:32x8x32"},     {2, 64, 4, 64, "pf-nt:64x4x64"},  {2, 32, 8, 64, "pf-nt:32x8x64"},
 };
 constexpr int kNumTiles = sizeof(kTiles) / sizeof(kTiles[0]);
 
@@ -800,21 +818,25 @@ static void launch_tile(const Fields<float>& f, const BCDesc<float>& bc, const P
     bool periodic = true;
     for (int face = 0; face < 6; ++face)
         periodic = periodic && bc.qkind[face] == QB_PERIODIC && bc.ukind[face] == UB_PERIODIC;
-    if (KIND == 0) {
+    const bool up = k.advection != 0;
+#define LBM_FUSED_PF(P, U, N)                                                                              \
+    hipLaunchKernelGGL((fused_step_kernel_pf<TX, TY, ZC, P, 2, U, N>), dim3(grid), dim3(TX * TY), 0, s, f, bc, k, \
+                       pull, aux, ntx, nty, nzc)
+    if constexpr (KIND == 0) {
         if (periodic)
             hipLaunchKernelGGL((fused_step_kernel<TX, TY, ZC, true>), dim3(grid), dim3(TX * TY), 0, s, f, bc, k,
                                pull, aux, ntx, nty, nzc);
         else
             hipLaunchKernelGGL((fused_step_kernel<TX, TY, ZC, false>), dim3(grid), dim3(TX * TY), 0, s, f, bc, k,
                                pull, aux, ntx, nty, nzc);
+    } else if constexpr (KIND == 1) {
+        if (periodic) { if (up) LBM_FUSED_PF(true, true, false); else LBM_FUSED_PF(true, false, false); }
+        else { if (up) LBM_FUSED_PF(false, true, false); else LBM_FUSED_PF(false, false, false); }
     } else {
-        if (periodic)
-            hipLaunchKernelGGL((fused_step_kernel_pf<TX, TY, ZC, true, 2>), dim3(grid), dim3(TX * TY), 0, s, f, bc,
-                               k, pull, aux, ntx, nty, nzc);
-        else
-            hipLaunchKernelGGL((fused_step_kernel_pf<TX, TY, ZC, false, 2>), dim3(grid), dim3(TX * TY), 0, s, f, bc,
-                               k, pull, aux, ntx, nty, nzc);
+        if (periodic) { if (up) LBM_FUSED_PF(true, true, true); else LBM_FUSED_PF(true, false, true); }
+        else { if (up) LBM_FUSED_PF(false, true, true); else LBM_FUSED_PF(false, false, true); }
     }
+#undef LBM_FUSED_PF
 }
 
 void launch_fused_step_f32(const Fields<float>& f, const BCDesc<float>& bc, const Phys<float>& k, int tile,
@@ -822,26 +844,37 @@ void launch_fused_step_f32(const Fields<float>& f, const BCDesc<float>& bc, cons
     switch (tile) {
         case 0: launch_tile<0, 32, 8, 32>(f, bc, k, pull, aux, s); break;
         case 1: launch_tile<0, 32, 8, 64>(f, bc, k, pull, aux, s); break;
-        case 2: launch_tile<0, 32, 8, 16>(f, bc, k, pull, aux, s); break;
-        case 3: launch_tile<0, 64, 4, 32>(f, bc, k, pull, aux, s); break;
-        case 4: launch_tile<0, 16, 16, 32>(f, bc, k, pull, aux, s); break;
-        case 5: launch_tile<0, 16, 16, 64>(f, bc, k, pull, aux, s); break;
-        case 6: launch_tile<1, 32, 8, 64>(f, bc, k, pull, aux, s); break;
-        case 7: launch_tile<1, 32, 8, 32>(f, bc, k, pull, aux, s); break;
-        case 8: launch_tile<1, 16, 16, 64>(f, bc, k, pull, aux, s); break;
-        default: launch_tile<1, 64, 4, 64>(f, bc, k, pull, aux, s); break;
+        case 2: launch_tile<0, 64, 4, 32>(f, bc, k, pull, aux, s); break;
+        case 3: launch_tile<1, 64, 4, 64>(f, bc, k, pull, aux, s); break;
+        case 4: launch_tile<1, 32, 8, 64>(f, bc, k, pull, aux, s); break;
+        case 5: launch_tile<1, 64, 4, 32>(f, bc, k, pull, aux, s); break;
+        case 6: launch_tile<1, 32, 8, 32>(f, bc, k, pull, aux, s); break;
+        case 7: launch_tile<2, 64, 4, 64>(f, bc, k, pull, aux, s); break;
+        default: launch_tile<2, 32, 8, 64>(f, bc, k, pull, aux, s); break;
     }
 }
 
 // ---- HBM calibration: plain 16-byte-per-lane copy ------------------------------
-__global__ void __launch_bounds__(256) copy_kernel(const float4* __restrict__ src, float4* __restrict__ dst,
-                                                   long n4) {
-    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += (long)gridDim.x * blockDim.x)
-        dst[i] = src[i];
+typedef float v4f __attribute__((ext_vector_type(4)));
+__global__ void __launch_bounds__(256) copy_kernel(const v4f* __restrict__ src, v4f* __restrict__ dst, long n4) {
+    // Four independent 16-byte loads in flight per lane, then the stores.
+    const long base = ((long)blockIdx.x * 4) * blockDim.x + threadIdx.x;
+    v4f v[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const long i = base + (long)r * blockDim.x;
+        if (i < n4) v[r] = __builtin_nontemporal_load(&src[i]);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const long i = base + (long)r * blockDim.x;
+        if (i < n4) __builtin_nontemporal_store(v[r], &dst[i]);
+    }
 }
 void launch_copy(const void* src, void* dst, size_t bytes, hipStream_t s) {
     const long n4 = (long)(bytes / 16);
-    hipLaunchKernelGGL(copy_kernel, dim3(256 * 8 * 4), dim3(256), 0, s, (const float4*)src, (float4*)dst, n4);
+    const long blocks = (n4 + 4L * 256 - 1) / (4L * 256);
+    hipLaunchKernelGGL(copy_kernel, dim3((unsigned)blocks), dim3(256), 0, s, (const v4f*)src, (v4f*)dst, n4);
 }
 
 }  // namespace lbm
