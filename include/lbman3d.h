@@ -229,6 +229,13 @@ int lbm_batch_download_state(lbm_batch_handle b, int32_t sim, double* rho,
 int lbm_batch_step(lbm_batch_handle b, int32_t n, int32_t use_graph);
 int lbm_batch_time_steps(lbm_batch_handle b, int32_t n, int32_t use_graph, double* ms);
 int lbm_batch_diagnostics(lbm_batch_handle b, int32_t sim, lbm_diag* out);
+/* Tile shapes of the batched kernel: count, select, query, and an autotuner
+ * that times every shape on scratch copies of the states and keeps the
+ * fastest (ms per step in ms_out[count]). */
+int lbm_batch_tile_count(void);
+int lbm_batch_set_tile(lbm_batch_handle b, int32_t tile);
+int lbm_batch_get_tile(lbm_batch_handle b, int32_t* tile);
+int lbm_batch_autotune(lbm_batch_handle b, int32_t n, double* ms_out, int32_t max, int32_t* count);
 
 #ifdef __cplusplus
 }

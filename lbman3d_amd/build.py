@@ -22,7 +22,7 @@ SOURCES = {
     "capi.cpp": ["-x", "hip", "-ffp-contract=off"],
     "kernels_fp64.hip": ["-ffp-contract=off"],
     "kernels_fp32.hip": ["-ffp-contract=fast", "-munsafe-fp-atomics"],
-    "batch.hip": ["-ffp-contract=fast"],
+    "batch.hip": ["-ffp-contract=fast", "-x", "hip"],
 }
 EXTRA = ["closure.hip"]
 

@@ -1,13 +1,408 @@
-// batch.hip — placeholder for the batched multi-simulation form.
-#include "../../include/lbman3d.h"
-#include "runtime.hpp"
+// batch.hip — many independent FP32 simulations advanced by one persistent
+// kernel launch per step, optionally replayed from a hipGraph.
+//
+// The reference can run exactly one compile-time configuration per binary
+// (src/params.h, src/device_solver.cu:123-125); here every simulation of a
+// batch has its own grid, boundary tags and constants. Each simulation is an
+// ordinary FP32 handle (own HBM buffers, slot table, diagnostics); the batch
+// adds a device array of per-simulation descriptors and a work list of
+// (simulation, tile) items that a persistent grid of blocks walks each step.
+// A descriptor holds both ping-pong views of the simulation's buffers, so two
+// consecutive steps differ only in a parity argument and can be captured once
+// as a graph and replayed.
+#define LBM_FUSED_BODY_ONLY
+#include "step_kernels.hpp"
+#include "fused_step.hpp"
+#include "solver_state.hpp"
+
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <vector>
+
 using namespace lbm;
-extern "C" {
-int lbm_batch_create(const lbm_config*, int32_t, lbm_batch_handle* out) { if (out) *out = nullptr; return fail(LBM_ERR_UNSUPPORTED, "batched form not built"); }
-int lbm_batch_destroy(lbm_batch_handle) { return LBM_OK; }
-int lbm_batch_upload_state(lbm_batch_handle, int32_t, const double*, const double*, const double*, const double*, const double*) { return fail(LBM_ERR_UNSUPPORTED, "batched form not built"); }
-int lbm_batch_download_state(lbm_batch_handle, int32_t, double*, double*, double*, double*, double*) { return fail(LBM_ERR_UNSUPPORTED, "batched form not built"); }
-int lbm_batch_step(lbm_batch_handle, int32_t, int32_t) { return fail(LBM_ERR_UNSUPPORTED, "batched form not built"); }
-int lbm_batch_time_steps(lbm_batch_handle, int32_t, int32_t, double*) { return fail(LBM_ERR_UNSUPPORTED, "batched form not built"); }
-int lbm_batch_diagnostics(lbm_batch_handle, int32_t, lbm_diag*) { return fail(LBM_ERR_UNSUPPORTED, "batched form not built"); }
+
+namespace {
+
+struct SimDesc {
+    Fields<float> f[2];   // [parity]: parity 1 swaps Q/Q', u/u', g/g'
+    BCDesc<float> bc;
+    Phys<float> k;
+    int periodic;
+    int ntx, nty, nzc;
+    int tile_begin;       // first work item of this simulation
+    int pull;             // 0 until the first step after an upload
+    int aux_always;       // moving walls read rho every step
+};
+
+// Tile shapes of the batched kernel (first fused pipeline).
+struct BatchTile {
+    int tx, ty, zc;
+    const char* name;
+};
+constexpr BatchTile kBatchTiles[] = {{32, 8, 32, "32x8x32"}, {16, 16, 32, "16x16x32"}, {32, 8, 16, "32x8x16"},
+                                     {16, 16, 16, "16x16x16"}};
+constexpr int kNumBatchTiles = sizeof(kBatchTiles) / sizeof(kBatchTiles[0]);
+
+template <int TX, int TY, int ZC>
+__global__ void __launch_bounds__(TX * TY)
+batch_step_kernel(const SimDesc* __restrict__ descs, int nsims, int total, int parity, int aux) {
+    using G = TileGeom<TX, TY>;
+    __shared__ float sIn[kNumIn][G::NI];
+    __shared__ float sA[2][kNumA][G::NA];
+    __shared__ float sRing[3][kNumIn][G::NR];
+    // Persistent blocks: every block walks the work list with a grid stride;
+    // the exit condition (w >= total) is reached by every block.
+    for (int w = blockIdx.x; w < total; w += gridDim.x) {
+        int lo = 0, hi = nsims - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (descs[mid].tile_begin <= w) lo = mid;
+            else hi = mid - 1;
+        }
+        const SimDesc& d = descs[lo];
+        const int local = w - d.tile_begin;
+        const int tx = local % d.ntx, ty = (local / d.ntx) % d.nty, tz = local / (d.ntx * d.nty);
+        const int a = aux | d.aux_always;
+        if (d.periodic)
+            fused_body<TX, TY, ZC, true>(d.f[parity], d.bc, d.k, d.pull, a, tx, ty, tz, sIn, sA, sRing);
+        else
+            fused_body<TX, TY, ZC, false>(d.f[parity], d.bc, d.k, d.pull, a, tx, ty, tz, sIn, sA, sRing);
+        __syncthreads();
+    }
 }
+
+template <int TX, int TY, int ZC>
+int occupancy_blocks() {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, batch_step_kernel<TX, TY, ZC>, TX * TY, 0) != hipSuccess)
+        return 1;
+    return std::max(1, nb);
+}
+
+template <int TX, int TY, int ZC>
+void launch_batch(const SimDesc* d, int nsims, int total, int parity, int aux, int grid, hipStream_t s) {
+    hipLaunchKernelGGL((batch_step_kernel<TX, TY, ZC>), dim3(grid), dim3(TX * TY), 0, s, d, nsims, total, parity,
+                       aux);
+}
+
+}  // namespace
+
+struct lbm_batch {
+    std::vector<lbm_handle> sims;
+    std::vector<SimDesc> desc;
+    SimDesc* d_desc = nullptr;
+    hipStream_t stream = nullptr;
+    int total = 0;
+    int parity = 0;        // how far the buffers have ping-ponged since the handles were synced
+    int tile = 0;
+    int grid = 0;
+    bool desc_dirty = true;
+    hipGraphExec_t graph[2] = {nullptr, nullptr};   // two steps starting at parity 0 / 1
+};
+
+namespace {
+
+void drop_graphs(lbm_batch* b) {
+    for (auto& g : b->graph)
+        if (g) {
+            (void)hipGraphExecDestroy(g);
+            g = nullptr;
+        }
+}
+
+int tiles_of(const lbm_batch* b, int i, int& ntx, int& nty, int& nzc) {
+    const BatchTile& t = kBatchTiles[b->tile];
+    const lbm_config& c = b->sims[i]->cfg;
+    ntx = (c.nx + t.tx - 1) / t.tx;
+    nty = (c.ny + t.ty - 1) / t.ty;
+    nzc = (c.nz + t.zc - 1) / t.zc;
+    return ntx * nty * nzc;
+}
+
+int grid_for_tile(int tile, int total) {
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) == hipSuccess) {
+        hipDeviceProp_t p;
+        if (hipGetDeviceProperties(&p, dev) == hipSuccess) cus = p.multiProcessorCount;
+    }
+    int occ = 1;
+    switch (tile) {
+        case 0: occ = occupancy_blocks<32, 8, 32>(); break;
+        case 1: occ = occupancy_blocks<16, 16, 32>(); break;
+        case 2: occ = occupancy_blocks<32, 8, 16>(); break;
+        default: occ = occupancy_blocks<16, 16, 16>(); break;
+    }
+    return std::max(1, std::min(total, cus * occ));
+}
+
+// Build the descriptors from the handles' current buffers (parity 0 view).
+int rebuild(lbm_batch* b) {
+    b->desc.resize(b->sims.size());
+    int begin = 0;
+    for (size_t i = 0; i < b->sims.size(); ++i) {
+        lbm_solver* s = b->sims[i];
+        SimDesc& d = b->desc[i];
+        d.f[0] = s->d32;
+        d.f[1] = s->d32;
+        for (int m = 0; m < 5; ++m) std::swap(d.f[1].q[m], d.f[1].qn[m]);
+        for (int m = 0; m < 3; ++m) std::swap(d.f[1].u[m], d.f[1].un[m]);
+        std::swap(d.f[1].g, d.f[1].gn);
+        d.bc = s->bc32;
+        d.k = s->k32;
+        d.periodic = 1;
+        for (int f = 0; f < 6; ++f)
+            if (s->cfg.walls[f].qbc != LBM_Q_PERIODIC || s->cfg.walls[f].ubc != LBM_U_PERIODIC) d.periodic = 0;
+        d.tile_begin = begin;
+        begin += tiles_of(b, (int)i, d.ntx, d.nty, d.nzc);
+        d.pull = s->pulled;
+        d.aux_always = s->moving ? 1 : 0;
+    }
+    b->total = begin;
+    b->grid = grid_for_tile(b->tile, b->total);
+    if (hipMemcpyAsync(b->d_desc, b->desc.data(), b->desc.size() * sizeof(SimDesc), hipMemcpyHostToDevice,
+                       b->stream) != hipSuccess)
+        return fail(LBM_ERR_HIP, "descriptor upload failed");
+    if (hipStreamSynchronize(b->stream) != hipSuccess) return fail(LBM_ERR_HIP, "descriptor upload failed");
+    b->desc_dirty = false;
+    drop_graphs(b);
+    return LBM_OK;
+}
+
+// Bring every handle's buffer pointers in line with the batch parity so
+// per-simulation calls (upload, download, diagnostics) see the current state.
+int sync_handles(lbm_batch* b) {
+    if (hipStreamSynchronize(b->stream) != hipSuccess) return fail(LBM_ERR_HIP, "batch synchronize failed");
+    if (b->parity) {
+        for (lbm_solver* s : b->sims) {
+            for (int m = 0; m < 5; ++m) std::swap(s->d32.q[m], s->d32.qn[m]);
+            for (int m = 0; m < 3; ++m) std::swap(s->d32.u[m], s->d32.un[m]);
+            std::swap(s->d32.g, s->d32.gn);
+        }
+        b->parity = 0;
+        b->desc_dirty = true;
+    }
+    return LBM_OK;
+}
+
+void launch_step(lbm_batch* b, int aux) {
+    const int n = (int)b->sims.size();
+    switch (b->tile) {
+        case 0: launch_batch<32, 8, 32>(b->d_desc, n, b->total, b->parity, aux, b->grid, b->stream); break;
+        case 1: launch_batch<16, 16, 32>(b->d_desc, n, b->total, b->parity, aux, b->grid, b->stream); break;
+        case 2: launch_batch<32, 8, 16>(b->d_desc, n, b->total, b->parity, aux, b->grid, b->stream); break;
+        default: launch_batch<16, 16, 16>(b->d_desc, n, b->total, b->parity, aux, b->grid, b->stream); break;
+    }
+}
+
+// One step outside any graph; afterwards every simulation pulls.
+int plain_step(lbm_batch* b, int aux) {
+    if (b->desc_dirty)
+        if (int r = rebuild(b)) return r;
+    launch_step(b, aux);
+    if (hipGetLastError() != hipSuccess) return fail(LBM_ERR_HIP, "batch step launch failed");
+    b->parity ^= 1;
+    bool any_fresh = false;
+    for (auto& d : b->desc) any_fresh = any_fresh || d.pull == 0;
+    if (any_fresh) {
+        for (lbm_solver* s : b->sims) s->pulled = 1;
+        for (auto& d : b->desc) d.pull = 1;
+        if (hipMemcpyAsync(b->d_desc, b->desc.data(), b->desc.size() * sizeof(SimDesc), hipMemcpyHostToDevice,
+                           b->stream) != hipSuccess)
+            return fail(LBM_ERR_HIP, "descriptor upload failed");
+        drop_graphs(b);
+    }
+    return LBM_OK;
+}
+
+int graph_pair(lbm_batch* b) {
+    hipGraphExec_t& ge = b->graph[b->parity];
+    if (!ge) {
+        hipGraph_t g = nullptr;
+        if (hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
+            return fail(LBM_ERR_HIP, "graph capture failed to start");
+        launch_step(b, 0);
+        b->parity ^= 1;
+        launch_step(b, 0);
+        b->parity ^= 1;
+        if (hipStreamEndCapture(b->stream, &g) != hipSuccess) return fail(LBM_ERR_HIP, "graph capture failed");
+        const hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) {
+            ge = nullptr;
+            return fail(LBM_ERR_HIP, "graph instantiate failed");
+        }
+    }
+    if (hipGraphLaunch(ge, b->stream) != hipSuccess) return fail(LBM_ERR_HIP, "graph launch failed");
+    return LBM_OK;   // two steps: parity unchanged
+}
+
+int batch_steps(lbm_batch* b, int n, int use_graph) {
+    if (n <= 0) return LBM_OK;
+    int done = 0;
+    if (b->desc_dirty)
+        if (int r = rebuild(b)) return r;
+    bool fresh = false;
+    for (auto& d : b->desc) fresh = fresh || d.pull == 0;
+    if (fresh) {
+        if (int r = plain_step(b, n == 1)) return r;
+        done = 1;
+    }
+    if (use_graph) {
+        while (n - done > 2) {
+            if (int r = graph_pair(b)) return r;
+            done += 2;
+        }
+    }
+    while (done < n) {
+        if (int r = plain_step(b, done == n - 1)) return r;
+        ++done;
+    }
+    return LBM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lbm_batch_create(const lbm_config* cfgs, int32_t count, lbm_batch_handle* out) {
+    if (!cfgs || !out || count < 1) return fail(LBM_ERR_INVALID, "bad argument");
+    *out = nullptr;
+    for (int i = 0; i < count; ++i)
+        if (cfgs[i].precision != LBM_FP32_SHIFTED)
+            return fail(LBM_ERR_UNSUPPORTED, "batched simulations must use LBM_FP32_SHIFTED");
+    lbm_batch* b = new lbm_batch();
+    auto bail = [&](int r) {
+        lbm_batch_destroy(b);
+        return r;
+    };
+    for (int i = 0; i < count; ++i) {
+        lbm_handle h = nullptr;
+        if (int r = lbm_create(&cfgs[i], &h)) return bail(r);
+        b->sims.push_back(h);
+        if (!h->fused_ok) return bail(fail(LBM_ERR_UNSUPPORTED, "simulation too large for the batched kernel"));
+    }
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess)
+        return bail(fail(LBM_ERR_HIP, "hipStreamCreate failed"));
+    if (hipMalloc(&b->d_desc, count * sizeof(SimDesc)) != hipSuccess)
+        return bail(fail(LBM_ERR_HIP, "hipMalloc(descriptors) failed"));
+    if (int r = rebuild(b)) return bail(r);
+    *out = b;
+    return LBM_OK;
+}
+
+int lbm_batch_destroy(lbm_batch_handle b) {
+    if (!b) return LBM_OK;
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    drop_graphs(b);
+    for (lbm_handle h : b->sims) lbm_destroy(h);
+    if (b->d_desc) (void)hipFree(b->d_desc);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+    return LBM_OK;
+}
+
+int lbm_batch_upload_state(lbm_batch_handle b, int32_t sim, const double* rho, const double* ux, const double* uy,
+                           const double* uz, const double* q) {
+    if (!b || sim < 0 || sim >= (int)b->sims.size()) return fail(LBM_ERR_INVALID, "bad simulation index");
+    if (int r = sync_handles(b)) return r;
+    if (int r = lbm_upload_state(b->sims[sim], rho, ux, uy, uz, q)) return r;
+    b->desc_dirty = true;
+    return LBM_OK;
+}
+
+int lbm_batch_download_state(lbm_batch_handle b, int32_t sim, double* rho, double* ux, double* uy, double* uz,
+                             double* q) {
+    if (!b || sim < 0 || sim >= (int)b->sims.size()) return fail(LBM_ERR_INVALID, "bad simulation index");
+    if (int r = sync_handles(b)) return r;
+    return lbm_download_state(b->sims[sim], rho, ux, uy, uz, q);
+}
+
+int lbm_batch_diagnostics(lbm_batch_handle b, int32_t sim, lbm_diag* out) {
+    if (!b || sim < 0 || sim >= (int)b->sims.size()) return fail(LBM_ERR_INVALID, "bad simulation index");
+    if (int r = sync_handles(b)) return r;
+    return lbm_diagnostics(b->sims[sim], out);
+}
+
+int lbm_batch_step(lbm_batch_handle b, int32_t n, int32_t use_graph) {
+    if (!b || n < 0) return fail(LBM_ERR_INVALID, "bad argument");
+    return batch_steps(b, n, use_graph);
+}
+
+int lbm_batch_time_steps(lbm_batch_handle b, int32_t n, int32_t use_graph, double* ms) {
+    if (!b || !ms || n < 1) return fail(LBM_ERR_INVALID, "bad argument");
+    if (b->desc_dirty)
+        if (int r = rebuild(b)) return r;
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+        return fail(LBM_ERR_HIP, "hipEventCreate failed");
+    (void)hipEventRecord(e0, b->stream);
+    int r = batch_steps(b, n, use_graph);
+    (void)hipEventRecord(e1, b->stream);
+    (void)hipEventSynchronize(e1);
+    float f = 0.0f;
+    (void)hipEventElapsedTime(&f, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *ms = f;
+    return r;
+}
+
+int lbm_batch_tile_count(void) { return kNumBatchTiles; }
+
+int lbm_batch_set_tile(lbm_batch_handle b, int32_t tile) {
+    if (!b || tile < 0 || tile >= kNumBatchTiles) return fail(LBM_ERR_INVALID, "tile out of range");
+    if (int r = sync_handles(b)) return r;
+    b->tile = tile;
+    b->desc_dirty = true;
+    return LBM_OK;
+}
+
+int lbm_batch_autotune(lbm_batch_handle b, int32_t n, double* ms_out, int32_t max, int32_t* count) {
+    if (!b || n < 1) return fail(LBM_ERR_INVALID, "bad argument");
+    if (count) *count = 0;
+    if (int r = sync_handles(b)) return r;
+    // Tune on scratch copies of every simulation's state, then restore.
+    std::vector<std::vector<double>> saved(b->sims.size());
+    for (size_t i = 0; i < b->sims.size(); ++i) {
+        lbm_solver* s = b->sims[i];
+        const long V = s->V;
+        saved[i].resize(9 * V + 15 * V);
+        double* p = saved[i].data();
+        if (int r = lbm_download_state(s, p, p + V, p + 2 * V, p + 3 * V, p + 4 * V)) return r;
+        if (int r = lbm_download_populations(s, p + 9 * V)) return r;
+    }
+    double best = 1e300;
+    int best_t = 0;
+    for (int t = 0; t < kNumBatchTiles; ++t) {
+        if (int r = lbm_batch_set_tile(b, t)) return r;
+        double ms = 0.0;
+        if (int r = batch_steps(b, 2, 0)) return r;
+        if (int r = lbm_batch_time_steps(b, n, 1, &ms)) return r;
+        if (ms_out && t < max) ms_out[t] = ms / n;
+        if (ms < best) {
+            best = ms;
+            best_t = t;
+        }
+    }
+    if (int r = lbm_batch_set_tile(b, best_t)) return r;
+    for (size_t i = 0; i < b->sims.size(); ++i) {
+        lbm_solver* s = b->sims[i];
+        const long V = s->V;
+        double* p = saved[i].data();
+        if (int r = lbm_upload_state(s, p, p + V, p + 2 * V, p + 3 * V, p + 4 * V)) return r;
+        if (int r = lbm_upload_populations(s, p + 9 * V)) return r;
+        s->pulled = 0;
+    }
+    b->desc_dirty = true;
+    if (count) *count = std::min(kNumBatchTiles, (int)max);
+    return LBM_OK;
+}
+
+int lbm_batch_get_tile(lbm_batch_handle b, int32_t* tile) {
+    if (!b || !tile) return fail(LBM_ERR_INVALID, "null argument");
+    *tile = b->tile;
+    return LBM_OK;
+}
+
+}  // extern "C"

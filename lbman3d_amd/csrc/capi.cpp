@@ -20,6 +20,7 @@
 #include "common.hpp"
 #include "launch.hpp"
 #include "runtime.hpp"
+#include "solver_state.hpp"
 
 using namespace lbm;
 
@@ -218,35 +219,6 @@ void feq_host(double rho, double ux, double uy, double uz, double out[kQ]) {
 
 }  // namespace lbm
 
-// ---------------------------------------------------------------------------
-// solver handle
-// ---------------------------------------------------------------------------
-struct lbm_solver {
-    lbm_config cfg{};
-    long V = 0, vs = 0;
-    bool fp64 = true;
-    hipStream_t stream = nullptr;
-    std::vector<void*> allocs;
-    Fields<double> d64{};
-    Fields<float> d32{};
-    BCDesc<double> bc64{};
-    BCDesc<float> bc32{};
-    Phys<double> k64{};
-    Phys<float> k32{};
-    uint16_t* slots = nullptr;
-    double* uprev = nullptr;
-    double* partials = nullptr;
-    double* diag_out = nullptr;
-    int diag_blocks = 0;
-    int pulled = 0;        // 0: g holds post-streaming f; 1: g holds post-collision
-    int schedule = 0;      // FP32: 0 auto, 1 two-kernel, 2+k fused tile k
-    int tuned = 0;         // schedule picked by lbm_autotune for "auto" (0: not tuned)
-    bool fused_ok = false; // fused kernel usable (FP32, 32-bit offsets fit)
-    bool moving = false;
-
-    size_t esize() const { return fp64 ? sizeof(double) : sizeof(float); }
-    void* gptr() const { return fp64 ? (void*)d64.g : (void*)d32.g; }
-};
 
 namespace {
 

@@ -323,23 +323,16 @@ __device__ __forceinline__ float stage_value(const StageA& s, int m) {
     return m < 5 ? s.qn[m] : (m < 10 ? s.sg[m - 5] : s.ta[m - 10]);
 }
 
+// First pipeline, one tile: z planes [tz*ZC, tz*ZC + ZC) of the TX x TY
+// column tile (tx, ty). LDS buffers come from the calling kernel so that a
+// kernel instantiating several variants does not allocate LDS per variant.
 template <int TX, int TY, int ZC, bool PERIODIC>
-__global__ void __launch_bounds__(TX * TY)
-fused_step_kernel(Fields<float> f, BCDesc<float> bc, Phys<float> k, int pull, int aux, int ntx, int nty,
-                  int nzc) {
+__device__ __forceinline__ void fused_body(const Fields<float>& f, const BCDesc<float>& bc, const Phys<float>& k,
+                                           int pull, int aux, int tx, int ty, int tz,
+                                           float (*sIn)[TileGeom<TX, TY>::NI],
+                                           float (*sA)[kNumA][TileGeom<TX, TY>::NA],
+                                           float (*sRing)[kNumIn][TileGeom<TX, TY>::NR]) {
     using G = TileGeom<TX, TY>;
-    __shared__ float sIn[kNumIn][G::NI];          // input plane pa, extended region
-    __shared__ float sA[2][kNumA][G::NA];         // stage-A results, planes pa and pa - 1
-    __shared__ float sRing[3][kNumIn][G::NR];     // input history of the stage-A ring columns
-
-    // XCD-aware block order: hardware deals consecutive block ids round-robin
-    // over the 8 XCDs; renumber so each XCD gets a contiguous run of tiles
-    // (neighbouring tiles share halos in that XCD's L2).
-    const int nblk = ntx * nty * nzc;
-    const int per_xcd = (nblk + 7) / 8;
-    const int L = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
-    if (L >= nblk) return;
-    const int tx = L % ntx, ty = (L / ntx) % nty, tz = L / (ntx * nty);
     const int nx = bc.n[0], ny = bc.n[1], nz = bc.n[2];
     const int x0 = tx * TX, y0 = ty * TY;
     const int z0 = tz * ZC, z1 = min(z0 + ZC, nz);
@@ -570,6 +563,25 @@ fused_step_kernel(Fields<float> f, BCDesc<float> bc, Phys<float> k, int pull, in
     }
 }
 
+template <int TX, int TY, int ZC, bool PERIODIC>
+__global__ void __launch_bounds__(TX * TY)
+fused_step_kernel(Fields<float> f, BCDesc<float> bc, Phys<float> k, int pull, int aux, int ntx, int nty,
+                  int nzc) {
+    using G = TileGeom<TX, TY>;
+    __shared__ float sIn[kNumIn][G::NI];          // input plane pa, extended region
+    __shared__ float sA[2][kNumA][G::NA];         // stage-A results, planes pa and pa - 1
+    __shared__ float sRing[3][kNumIn][G::NR];     // input history of the stage-A ring columns
+    // XCD-aware block order: hardware deals consecutive block ids round-robin
+    // over the 8 XCDs; renumber so each XCD gets a contiguous run of tiles
+    // (neighbouring tiles share halos in that XCD's L2).
+    const int nblk = ntx * nty * nzc;
+    const int per_xcd = (nblk + 7) / 8;
+    const int L = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
+    if (L >= nblk) return;
+    const int tx = L % ntx, ty = (L / ntx) % nty, tz = L / (ntx * nty);
+    fused_body<TX, TY, ZC, PERIODIC>(f, bc, k, pull, aux, tx, ty, tz, sIn, sA, sRing);
+}
+
 // Second pipeline: the same computation with one stage-A plane in LDS (the
 // force stage reads plane pb while the fresh plane pa waits in registers and
 // is published after a barrier), and with every global load issued one z
@@ -791,6 +803,7 @@ fused_step_kernel_pf(Fields<float> f, BCDesc<float> bc, Phys<float> k, int pull,
     }
 }
 
+#ifndef LBM_FUSED_BODY_ONLY
 // ---- tile catalogue -----------------------------------------------------------
 // kind 0: first pipeline (stage-A planes double-buffered in LDS);
 // kind 1: prefetch pipeline (single stage-A plane, loads one plane ahead);
@@ -876,5 +889,7 @@ void launch_copy(const void* src, void* dst, size_t bytes, hipStream_t s) {
     const long blocks = (n4 + 4L * 256 - 1) / (4L * 256);
     hipLaunchKernelGGL(copy_kernel, dim3((unsigned)blocks), dim3(256), 0, s, (const v4f*)src, (v4f*)dst, n4);
 }
+
+#endif  // LBM_FUSED_BODY_ONLY
 
 }  // namespace lbm

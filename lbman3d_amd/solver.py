@@ -60,6 +60,10 @@ SIGNATURES = {
     "lbm_batch_step": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
     "lbm_batch_time_steps": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, _D]),
     "lbm_batch_diagnostics": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(LbmDiag)]),
+    "lbm_batch_tile_count": (C.c_int, []),
+    "lbm_batch_set_tile": (C.c_int, [C.c_void_p, C.c_int32]),
+    "lbm_batch_get_tile": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32)]),
+    "lbm_batch_autotune": (C.c_int, [C.c_void_p, C.c_int32, _D, C.c_int32, C.POINTER(C.c_int32)]),
 }
 
 
@@ -243,3 +247,73 @@ def schedule_name(s: int) -> str:
     buf = C.create_string_buffer(128)
     check(lib().lbm_tile_name(s, buf, 128))
     return buf.value.decode()
+
+
+class BatchSolver:
+    """Many independent FP32 simulations stepped by one launch per step
+    (lbm_batch_*). Every config must be FP32_SHIFTED."""
+
+    def __init__(self, cfgs):
+        self.cfgs = list(cfgs)
+        arr = (LbmConfig * len(self.cfgs))(*self.cfgs)
+        h = C.c_void_p()
+        check(lib().lbm_batch_create(arr, len(self.cfgs), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().lbm_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _V(self, i):
+        c = self.cfgs[i]
+        return c.nx * c.ny * c.nz
+
+    def upload_state(self, i, rho=None, u=None, q=None):
+        arr = lambda a: None if a is None else np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
+        rho, q = arr(rho), arr(q)
+        u = [arr(u[a]) for a in range(3)] if u is not None else [None] * 3
+        check(lib().lbm_batch_upload_state(self.h, i, _p(rho), _p(u[0]), _p(u[1]), _p(u[2]), _p(q)))
+
+    def state(self, i):
+        V = self._V(i)
+        rho, ux, uy, uz = (np.empty(V) for _ in range(4))
+        q = np.empty(5 * V)
+        check(lib().lbm_batch_download_state(self.h, i, _p(rho), _p(ux), _p(uy), _p(uz), _p(q)))
+        return {"rho": rho, "u": np.stack([ux, uy, uz]), "q": q.reshape(5, V)}
+
+    def diagnostics(self, i) -> dict:
+        d = LbmDiag()
+        check(lib().lbm_batch_diagnostics(self.h, i, C.byref(d)))
+        return d.as_dict()
+
+    def step(self, n=1, use_graph=True):
+        check(lib().lbm_batch_step(self.h, n, 1 if use_graph else 0))
+
+    def time_steps(self, n, use_graph=True) -> float:
+        ms = C.c_double()
+        check(lib().lbm_batch_time_steps(self.h, n, 1 if use_graph else 0, C.byref(ms)))
+        return ms.value
+
+    def set_tile(self, t):
+        check(lib().lbm_batch_set_tile(self.h, t))
+
+    def get_tile(self) -> int:
+        t = C.c_int32()
+        check(lib().lbm_batch_get_tile(self.h, C.byref(t)))
+        return t.value
+
+    def autotune(self, n=5) -> list:
+        ms = (C.c_double * 16)()
+        cnt = C.c_int32()
+        check(lib().lbm_batch_autotune(self.h, n, ms, 16, C.byref(cnt)))
+        return [ms[i] for i in range(cnt.value)]
