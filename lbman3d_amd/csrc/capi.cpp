@@ -720,9 +720,21 @@ int lbm_lbm_step(lbm_handle s) {
     return run_lbm(s, nullptr);
 }
 
+// The automatic FP32 schedule is tuned on first use: the best schedule
+// depends on the grid (small grids give the fused z-marching kernel too few
+// tiles to fill 256 CUs). Tuning runs on a snapshot and leaves the state as
+// it was; if it cannot run, the default fused tile is used.
+static void ensure_tuned(lbm_handle s) {
+    if (s->fp64 || s->schedule != 0 || s->tuned != 0) return;
+    const std::string saved_error = g_last_error;
+    if (lbm_autotune(s, 3, nullptr, 0, nullptr) != LBM_OK) s->tuned = s->fused_ok ? 2 : 1;
+    g_last_error = saved_error;
+}
+
 int lbm_step(lbm_handle s, int32_t n) {
     if (!s) return fail(LBM_ERR_INVALID, "null handle");
     if (n < 0) return fail(LBM_ERR_INVALID, "negative step count");
+    if (n > 0) ensure_tuned(s);
     for (int t = 0; t < n; ++t)
         if (int r = run_step(s, t == n - 1, nullptr)) return r;
     return LBM_OK;
@@ -777,6 +789,7 @@ int lbm_order_director(lbm_handle s, double* order, double* director) {
 
 int lbm_time_steps(lbm_handle s, int32_t n, double* ms) {
     if (!s || !ms || n < 1) return fail(LBM_ERR_INVALID, "bad argument");
+    ensure_tuned(s);
     hipEvent_t a, b;
     HIPCHK(hipEventCreate(&a));
     HIPCHK(hipEventCreate(&b));
