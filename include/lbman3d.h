@@ -237,6 +237,46 @@ int lbm_batch_set_tile(lbm_batch_handle b, int32_t tile);
 int lbm_batch_get_tile(lbm_batch_handle b, int32_t* tile);
 int lbm_batch_autotune(lbm_batch_handle b, int32_t n, double* ms_out, int32_t max, int32_t* count);
 
+/* ---- learned closure for a 2x-coarsened FP32 simulation -------------------
+ * No counterpart in the reference. A periodic 3x3x3 convolutional network
+ * 8 -> hidden -> hidden -> 5 (ReLU between layers) maps the coarse state
+ * x = (Q 5, u 3) to a correction dQ of the coarse step's Q. BF16 matrix-core
+ * kernels with FP32 accumulation; FP32 master weights and Adam state.
+ * Fields on the host are channels-first FP64: x is 8 blocks of V (Q xx, xy,
+ * xz, yy, yz, then ux, uy, uz), y is 5 blocks of V. nx must be a multiple of
+ * 16; hidden must be 32. Parameters, layer by layer: W[27][cin][cout] with
+ * stencil offset (dz+1)*9 + (dy+1)*3 + (dx+1), then b[cout]. */
+typedef struct lbm_closure* lbm_closure_handle;
+int lbm_closure_create(int32_t nx, int32_t ny, int32_t nz, int32_t hidden, uint64_t seed,
+                       lbm_closure_handle* out);
+int lbm_closure_destroy(lbm_closure_handle c);
+int lbm_closure_param_count(lbm_closure_handle c, int64_t* n);
+int lbm_closure_get_params(lbm_closure_handle c, double* w);
+/* Replaces the weights and resets the optimizer state. */
+int lbm_closure_set_params(lbm_closure_handle c, const double* w);
+/* Network input = x * in_scale[8]; network output = y * out_scale[5]. */
+int lbm_closure_set_normalization(lbm_closure_handle c, const double* in_scale, const double* out_scale);
+/* Training samples are kept on the device (BF16 inputs, FP32 targets). */
+int lbm_closure_add_sample(lbm_closure_handle c, const double* x, const double* y, int32_t* index);
+int lbm_closure_clear_samples(lbm_closure_handle c);
+/* iters Adam steps (beta 0.9/0.999, eps 1e-8) on the mean-squared error of
+ * one sample each, drawn from the seeded stream; mean_loss (may be NULL)
+ * is the mean normalised loss over the iterations. */
+int lbm_closure_train(lbm_closure_handle c, int32_t iters, double lr, uint64_t seed, double* mean_loss);
+/* Normalised loss and its gradient (param_count values, may be NULL) for
+ * one sample; does not change the weights. */
+int lbm_closure_loss_and_grad(lbm_closure_handle c, const double* x, const double* y, double* loss,
+                              double* grad);
+/* y = network(x) in physical units (the output scale undone). */
+int lbm_closure_predict(lbm_closure_handle c, const double* x, double* y);
+/* Adds alpha * network(Q, u) to the Q of an FP32 simulation of the same grid. */
+int lbm_closure_apply(lbm_closure_handle c, lbm_handle sim, double alpha);
+/* Mean time of one forward pass over the whole grid (ms), n passes. */
+int lbm_closure_time_forward(lbm_closure_handle c, int32_t n, double* ms);
+/* D = A (16x32, row-major) * B (32x16) through the BF16 matrix-core
+ * fragment mapping the closure kernels use (inputs rounded to BF16). */
+int lbm_debug_mfma_bf16(const float* a, const float* b, float* d);
+
 #ifdef __cplusplus
 }
 #endif

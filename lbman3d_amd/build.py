@@ -23,8 +23,9 @@ SOURCES = {
     "kernels_fp64.hip": ["-ffp-contract=off"],
     "kernels_fp32.hip": ["-ffp-contract=fast", "-munsafe-fp-atomics"],
     "batch.hip": ["-ffp-contract=fast", "-x", "hip"],
+    "closure.hip": ["-ffp-contract=fast"],
 }
-EXTRA = ["closure.hip"]
+EXTRA: list[str] = []
 
 
 def _hipcc() -> str:

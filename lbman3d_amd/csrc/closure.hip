@@ -1,0 +1,722 @@
+// closure.hip — learned closure for a 2x-coarsened step: a three-layer
+// periodic 3x3x3 convolutional network (8 -> H -> H -> 5 channels, ReLU
+// between layers) that maps the coarse state (Q 5, u 3) to a correction of
+// the coarse step's Q update. Inference and training (forward, input and
+// weight gradients, Adam) are hand-written kernels on the BF16 matrix cores
+// (v_mfma_f32_16x16x32_bf16) with FP32 accumulation; master weights and the
+// optimizer state are FP32.
+//
+// Layouts
+//   activations  BF16 [cell][C] (channels fastest), cell = (z*ny + y)*nx + x
+//   weights      FP32 master, layer by layer: W[off][ci][co] then b[co], with
+//                off = (dz+1)*9 + (dy+1)*3 + (dx+1) the stencil offset
+//   packed       BF16 B-operand fragments [kchunk][ntile][lane][8]
+// A 3x3x3 convolution is the GEMM out[cell][co] = sum_k A[cell][k] W[k][co]
+// with k = off*Cin + ci and A[cell][k] = in[cell + d(off)][ci] gathered with
+// periodic wrap. The input gradient is the same form with the flipped,
+// transposed weights W'[off][co][ci] = W[26-off][ci][co]; the weight
+// gradient is the GEMM dW[k][co] = sum_cell A[cell][k] dOut[cell][co].
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/lbman3d.h"
+#include "common.hpp"
+#include "runtime.hpp"
+#include "solver_state.hpp"
+
+using namespace lbm;
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+namespace {
+
+constexpr int kOffsets = 27;
+constexpr int kInC = 8;     // Q (5) + u (3)
+constexpr int kOutC = 5;    // correction of Q
+
+__device__ __forceinline__ __bf16 to_bf16(float v) { return (__bf16)v; }
+__device__ __forceinline__ float from_bf16(__bf16 v) { return (float)v; }
+
+__device__ __forceinline__ int wrap(int t, int n) { return t < 0 ? t + n : (t >= n ? t - n : t); }
+
+// Neighbour cell of (x, y, z) for stencil offset `off` on a periodic grid.
+__device__ __forceinline__ int nbr(int x, int y, int z, int off, int nx, int ny, int nz) {
+    const int dz = off / 9 - 1, dy = (off / 3) % 3 - 1, dx = off % 3 - 1;
+    return (wrap(z + dz, nz) * ny + wrap(y + dy, ny)) * nx + wrap(x + dx, nx);
+}
+
+// ---- forward convolution (also the input-gradient convolution) -------------
+// One wavefront: 16 consecutive cells x (16 * NT) output channels. Cin is 8
+// (four stencil offsets per 32-wide K step) or 32 (one offset per K step).
+// EPI: 0 = store BF16 with bias + ReLU, 1 = store BF16 linear (no bias),
+// 2 = store FP32 channels-first [co][cell] with bias (final layer).
+template <int CIN, int NT, int EPI>
+__global__ void __launch_bounds__(256)
+conv_fwd_kernel(const __bf16* __restrict__ in, const bf16x8* __restrict__ wpk, const float* __restrict__ bias,
+                void* __restrict__ out, int cout, int nx, int ny, int nz) {
+    const int lane = threadIdx.x & 63;
+    const long wave = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const long V = (long)nx * ny * nz;
+    const long c0 = wave * 16;
+    if (c0 >= V) return;
+    const int m = lane & 15, kb = lane >> 4;
+    const long cm = c0 + m;
+    const int x = (int)(cm % nx), y = (int)((cm / nx) % ny), z = (int)(cm / ((long)nx * ny));
+    constexpr int KCH = CIN == 8 ? 7 : 27;   // K steps of 32 (8-channel input: 28 offsets, last zero)
+    f32x4 acc[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) acc[t] = (f32x4){0.f, 0.f, 0.f, 0.f};
+    for (int kc = 0; kc < KCH; ++kc) {
+        bf16x8 a;
+        int off, ch;
+        if (CIN == 8) {
+            off = kc * 4 + kb;
+            ch = 0;
+        } else {
+            off = kc;
+            ch = kb * 8;
+        }
+        if (off < kOffsets) {
+            const int nc = nbr(x, y, z, off, nx, ny, nz);
+            a = *reinterpret_cast<const bf16x8*>(in + (long)nc * CIN + ch);
+        } else {
+#pragma unroll
+            for (int j = 0; j < 8; ++j) a[j] = to_bf16(0.f);
+        }
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            const bf16x8 b = wpk[((long)kc * NT + t) * 64 + lane];
+            acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[t], 0, 0, 0);
+        }
+    }
+    // D fragment: lane holds rows 4*kb + r (cells), column n = m (channel).
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+        const int co = t * 16 + m;
+        if (co >= cout) continue;
+        const float bco = (EPI == 1) ? 0.f : bias[co];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const long cell = c0 + 4 * kb + r;
+            float v = acc[t][r] + bco;
+            if (EPI == 0) v = v > 0.f ? v : 0.f;
+            if (EPI == 2) reinterpret_cast<float*>(out)[(long)co * V + cell] = v;
+            else reinterpret_cast<__bf16*>(out)[cell * cout + co] = to_bf16(v);
+        }
+    }
+}
+
+// ---- weight gradient --------------------------------------------------------
+// dW[k][co] = sum_cell A[cell][k] dOut[cell][co] for one 16 x 16 (k, co) tile
+// per wavefront over a chunk of cells; partial sums land in
+// part[chunk][k][co] and are reduced by wgrad_reduce_kernel.
+template <int CIN>
+__global__ void __launch_bounds__(256)
+conv_wgrad_kernel(const __bf16* __restrict__ in, const __bf16* __restrict__ dout, int cout_pad, int cout,
+                  float* __restrict__ part, int ktiles, int ntiles, int chunk, int nx, int ny, int nz) {
+    const int lane = threadIdx.x & 63;
+    const long wave = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const long V = (long)nx * ny * nz;
+    const int nchunks = (int)((V + chunk - 1) / chunk);
+    const long tiles = (long)ktiles * ntiles;
+    if (wave >= tiles * nchunks) return;
+    const int ch_id = (int)(wave / tiles);
+    const int tile = (int)(wave % tiles);
+    const int kt = tile / ntiles, nt = tile % ntiles;
+    const int m = lane & 15, kb = lane >> 4;
+    // A: row = k index kt*16 + m, 8 consecutive cells; B: column co = nt*16 + m.
+    const int k = kt * 16 + m;
+    const int off = k / CIN, ci = k % CIN;
+    const int co = nt * 16 + m;
+    f32x4 acc = (f32x4){0.f, 0.f, 0.f, 0.f};
+    const long cbeg = (long)ch_id * chunk, cend = std::min<long>(cbeg + chunk, V);
+    for (long cb = cbeg; cb < cend; cb += 32) {
+        bf16x8 a, b;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const long cell = cb + 8 * kb + j;
+            float av = 0.f, bv = 0.f;
+            if (cell < cend) {
+                const int x = (int)(cell % nx), y = (int)((cell / nx) % ny), z = (int)(cell / ((long)nx * ny));
+                if (off < kOffsets) av = from_bf16(in[(long)nbr(x, y, z, off, nx, ny, nz) * CIN + ci]);
+                if (co < cout) bv = from_bf16(dout[cell * cout_pad + co]);
+            }
+            a[j] = to_bf16(av);
+            b[j] = to_bf16(bv);
+        }
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
+    }
+    // D: rows k = kt*16 + 4*kb + r, column co.
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int kr = kt * 16 + 4 * kb + r;
+        part[((long)ch_id * (ktiles * 16) + kr) * (ntiles * 16) + co] = acc[r];
+    }
+}
+
+__global__ void wgrad_reduce_kernel(const float* __restrict__ part, int nchunks, int krows, int ncols, int kvalid,
+                                    int cvalid, float* __restrict__ grad) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= kvalid * cvalid) return;
+    const int k = i / cvalid, co = i % cvalid;
+    float s = 0.f;
+    for (int c = 0; c < nchunks; ++c) s += part[((long)c * krows + k) * ncols + co];
+    grad[i] = s;   // [k][co] = [off][ci][co]
+}
+
+// Bias gradient: sum over cells of dOut[cell][co].
+__global__ void bias_grad_kernel(const __bf16* __restrict__ dout, int cout_pad, int cout, long V,
+                                 float* __restrict__ grad) {
+    __shared__ float red[256];
+    const int co = blockIdx.x;
+    float s = 0.f;
+    for (long c = threadIdx.x; c < V; c += blockDim.x) s += from_bf16(dout[c * cout_pad + co]);
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) grad[co] = red[0];
+}
+
+// Mean-squared loss over V cells x 5 channels: dL/dpred = 2 (pred - y) / (5 V),
+// written as BF16 [cell][8] (channels 5..7 zero) for the backward convolutions;
+// per-block partial sums of the squared error.
+__global__ void __launch_bounds__(256)
+loss_grad_kernel(const float* __restrict__ pred, const float* __restrict__ target, long V, __bf16* __restrict__ dout,
+                 float* __restrict__ partial) {
+    __shared__ float red[256];
+    float s = 0.f;
+    const float scale = 2.0f / (kOutC * (float)V);
+    for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c < V; c += (long)gridDim.x * blockDim.x) {
+#pragma unroll
+        for (int co = 0; co < 8; ++co) {
+            float g = 0.f;
+            if (co < kOutC) {
+                const float d = pred[co * V + c] - target[co * V + c];
+                s += d * d;
+                g = scale * d;
+            }
+            dout[c * 8 + co] = to_bf16(g);
+        }
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+// ReLU backward in place: gradient is zero where the stored activation is zero.
+__global__ void relu_back_kernel(__bf16* __restrict__ g, const __bf16* __restrict__ act, long n) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && !(from_bf16(act[i]) > 0.f)) g[i] = to_bf16(0.f);
+}
+
+// Pack FP32 master weights W[off][ci][co] into B fragments for a
+// convolution kernel whose input has kcin channels per cell (K index
+// off * kcin + c). transpose: pack the input-gradient weights
+// W'[off][co][ci] = W[26 - off][ci][co].
+__global__ void pack_kernel(const float* __restrict__ w, int cin, int cout, int transpose, int kcin, int kch,
+                            int ntiles, bf16x8* __restrict__ out) {
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;   // (kc, t, lane)
+    if (idx >= (long)kch * ntiles * 64) return;
+    const int lane = (int)(idx % 64), t = (int)((idx / 64) % ntiles), kc = (int)(idx / (64 * ntiles));
+    const int ecin = transpose ? cout : cin, ecout = transpose ? cin : cout;
+    const int n = t * 16 + (lane & 15);
+    bf16x8 v;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        const int k = kc * 32 + 8 * (lane >> 4) + j;
+        const int off = k / kcin, c = k % kcin;
+        float val = 0.f;
+        if (off < kOffsets && c < ecin && n < ecout) {
+            val = transpose ? w[((long)(26 - off) * cin + n) * cout + c] : w[((long)off * cin + c) * cout + n];
+        }
+        v[j] = to_bf16(val);
+    }
+    out[idx] = v;
+}
+
+__global__ void adam_kernel(float* __restrict__ w, const float* __restrict__ g, float* __restrict__ m,
+                            float* __restrict__ v, long n, float lr, float b1, float b2, float eps, float c1,
+                            float c2) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.f - b1) * gi;
+    const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    w[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+// Closure input from a simulation's FP32 fields: x[cell][8] = scale * (Q, u).
+__global__ void gather_input_kernel(Fields<float> f, long V, const float* __restrict__ in_scale,
+                                    __bf16* __restrict__ x) {
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= V) return;
+#pragma unroll
+    for (int m = 0; m < 5; ++m) x[c * 8 + m] = to_bf16(f.q[m][c] * in_scale[m]);
+#pragma unroll
+    for (int m = 0; m < 3; ++m) x[c * 8 + 5 + m] = to_bf16(f.u[m][c] * in_scale[5 + m]);
+}
+
+// Q += pred / out_scale (apply the correction to a running simulation).
+__global__ void apply_kernel(Fields<float> f, long V, const float* __restrict__ pred, const float* __restrict__ out_scale,
+                             float alpha) {
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= V) return;
+#pragma unroll
+    for (int m = 0; m < 5; ++m) f.q[m][c] += alpha * pred[m * V + c] / out_scale[m];
+}
+
+// MFMA lane-layout check: D = A (16x32) * B (32x16) through the same
+// fragment mapping the convolutions use.
+__global__ void mfma_check_kernel(const float* a, const float* b, float* d) {
+    const int lane = threadIdx.x;
+    const int m = lane & 15, kb = lane >> 4;
+    bf16x8 av, bv;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        av[j] = to_bf16(a[m * 32 + 8 * kb + j]);
+        bv[j] = to_bf16(b[(8 * kb + j) * 16 + m]);
+    }
+    f32x4 acc = (f32x4){0.f, 0.f, 0.f, 0.f};
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc, 0, 0, 0);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) d[(4 * kb + r) * 16 + m] = acc[r];
+}
+
+unsigned blocks_for(long n, int bs = 256) { return (unsigned)((n + bs - 1) / bs); }
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+struct lbm_closure {
+    int nx = 0, ny = 0, nz = 0, hidden = 32;
+    long V = 0;
+    hipStream_t stream = nullptr;
+    std::vector<void*> allocs;
+    // parameters: offsets into the flat FP32 arrays
+    long off_w[3], off_b[3], nparams = 0;
+    int cin[3], cout[3];
+    float *w = nullptr, *g = nullptr, *m = nullptr, *v = nullptr;
+    bf16x8* pk[3] = {nullptr, nullptr, nullptr};    // forward B fragments
+    bf16x8* pkt[2] = {nullptr, nullptr};            // input-gradient B fragments of layers 2, 3
+    int kch[3], nt[3], kcht[2], ntt[2];
+    __bf16 *x = nullptr, *a1 = nullptr, *a2 = nullptr;   // activations [cell][C]
+    __bf16 *d3 = nullptr, *d2 = nullptr, *d1 = nullptr;   // gradients [cell][C]
+    float* pred = nullptr;                                // [5][cell]
+    float* part = nullptr;                                // weight-gradient partials
+    long part_elems = 0;
+    float* loss_part = nullptr;
+    int loss_blocks = 0;
+    float* in_scale = nullptr;
+    float* out_scale = nullptr;
+    double h_in_scale[8] = {1, 1, 1, 1, 1, 1, 1, 1}, h_out_scale[5] = {1, 1, 1, 1, 1};
+    std::vector<__bf16*> sx;   // stored samples: inputs
+    std::vector<float*> sy;    // stored samples: targets [5][cell] normalised
+    long step = 0;
+    int chunk = 4096;
+};
+
+namespace {
+
+#define CHK(expr)                                                                                \
+    do {                                                                                         \
+        hipError_t e_ = (expr);                                                                  \
+        if (e_ != hipSuccess) return fail(LBM_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+template <typename T>
+int cl_alloc(lbm_closure* c, T** p, size_t elems) {
+    void* q = nullptr;
+    CHK(hipMalloc(&q, std::max<size_t>(elems, 1) * sizeof(T)));
+    c->allocs.push_back(q);
+    CHK(hipMemsetAsync(q, 0, std::max<size_t>(elems, 1) * sizeof(T), c->stream));
+    *p = (T*)q;
+    return LBM_OK;
+}
+
+int repack(lbm_closure* c) {
+    for (int l = 0; l < 3; ++l) {
+        const long n = (long)c->kch[l] * c->nt[l] * 64;
+        hipLaunchKernelGGL(pack_kernel, dim3(blocks_for(n)), dim3(256), 0, c->stream, c->w + c->off_w[l], c->cin[l],
+                           c->cout[l], 0, c->cin[l], c->kch[l], c->nt[l], c->pk[l]);
+    }
+    for (int l = 1; l < 3; ++l) {
+        const long n = (long)c->kcht[l - 1] * c->ntt[l - 1] * 64;
+        // input of the transposed convolution: the layer's output gradient,
+        // stored with 8 channels (padded) for the last layer, else cout
+        const int kcin = c->cout[l] <= 8 ? 8 : c->cout[l];
+        hipLaunchKernelGGL(pack_kernel, dim3(blocks_for(n)), dim3(256), 0, c->stream, c->w + c->off_w[l], c->cin[l],
+                           c->cout[l], 1, kcin, c->kcht[l - 1], c->ntt[l - 1], c->pkt[l - 1]);
+    }
+    CHK(hipGetLastError());
+    return LBM_OK;
+}
+
+// Forward pass of the input x (BF16 [cell][8]) into c->pred.
+int forward(lbm_closure* c, const __bf16* x) {
+    const unsigned grid = blocks_for(c->V / 16 * 64);
+    const int H = c->hidden;
+    hipLaunchKernelGGL((conv_fwd_kernel<8, 2, 0>), dim3(grid), dim3(256), 0, c->stream, x, c->pk[0],
+                       c->w + c->off_b[0], (void*)c->a1, H, c->nx, c->ny, c->nz);
+    hipLaunchKernelGGL((conv_fwd_kernel<32, 2, 0>), dim3(grid), dim3(256), 0, c->stream, c->a1, c->pk[1],
+                       c->w + c->off_b[1], (void*)c->a2, H, c->nx, c->ny, c->nz);
+    hipLaunchKernelGGL((conv_fwd_kernel<32, 1, 2>), dim3(grid), dim3(256), 0, c->stream, c->a2, c->pk[2],
+                       c->w + c->off_b[2], (void*)c->pred, kOutC, c->nx, c->ny, c->nz);
+    CHK(hipGetLastError());
+    return LBM_OK;
+}
+
+int wgrad(lbm_closure* c, const __bf16* in, int cin, const __bf16* dout, int cout_pad, int cout, float* gw) {
+    const int krows = ((cin == 8 ? 28 : 27) * cin + 15) / 16 * 16;
+    const int ktiles = krows / 16, ntiles = (cout + 15) / 16;
+    const int nchunks = (int)((c->V + c->chunk - 1) / c->chunk);
+    const long waves = (long)ktiles * ntiles * nchunks;
+    if ((long)nchunks * krows * ntiles * 16 > c->part_elems) return fail(LBM_ERR_STATE, "wgrad scratch too small");
+    if (cin == 8)
+        hipLaunchKernelGGL((conv_wgrad_kernel<8>), dim3(blocks_for(waves * 64)), dim3(256), 0, c->stream, in, dout,
+                           cout_pad, cout, c->part, ktiles, ntiles, c->chunk, c->nx, c->ny, c->nz);
+    else
+        hipLaunchKernelGGL((conv_wgrad_kernel<32>), dim3(blocks_for(waves * 64)), dim3(256), 0, c->stream, in, dout,
+                           cout_pad, cout, c->part, ktiles, ntiles, c->chunk, c->nx, c->ny, c->nz);
+    const int kvalid = 27 * cin;
+    hipLaunchKernelGGL(wgrad_reduce_kernel, dim3(blocks_for((long)kvalid * cout)), dim3(256), 0, c->stream, c->part,
+                       nchunks, krows, ntiles * 16, kvalid, cout, gw);
+    CHK(hipGetLastError());
+    return LBM_OK;
+}
+
+// Forward + backward for one sample; gradient into c->g, loss (sum of
+// squared errors / (5V)) returned through loss_out (synchronises).
+int loss_and_grad(lbm_closure* c, const __bf16* x, const float* y, double* loss_out) {
+    if (int r = forward(c, x)) return r;
+    const int H = c->hidden;
+    const long V = c->V;
+    hipLaunchKernelGGL(loss_grad_kernel, dim3(c->loss_blocks), dim3(256), 0, c->stream, c->pred, y, V, c->d3,
+                       c->loss_part);
+    // layer 3: weights, bias, input gradient (8-channel padded gradient in, H out)
+    if (int r = wgrad(c, c->a2, H, c->d3, 8, kOutC, c->g + c->off_w[2])) return r;
+    hipLaunchKernelGGL(bias_grad_kernel, dim3(kOutC), dim3(256), 0, c->stream, c->d3, 8, kOutC, V,
+                       c->g + c->off_b[2]);
+    const unsigned grid = blocks_for(V / 16 * 64);
+    hipLaunchKernelGGL((conv_fwd_kernel<8, 2, 1>), dim3(grid), dim3(256), 0, c->stream, c->d3, c->pkt[1], nullptr,
+                       (void*)c->d2, H, c->nx, c->ny, c->nz);
+    hipLaunchKernelGGL(relu_back_kernel, dim3(blocks_for(V * H)), dim3(256), 0, c->stream, c->d2, c->a2, V * H);
+    // layer 2
+    if (int r = wgrad(c, c->a1, H, c->d2, H, H, c->g + c->off_w[1])) return r;
+    hipLaunchKernelGGL(bias_grad_kernel, dim3(H), dim3(256), 0, c->stream, c->d2, H, H, V, c->g + c->off_b[1]);
+    hipLaunchKernelGGL((conv_fwd_kernel<32, 2, 1>), dim3(grid), dim3(256), 0, c->stream, c->d2, c->pkt[0], nullptr,
+                       (void*)c->d1, H, c->nx, c->ny, c->nz);
+    hipLaunchKernelGGL(relu_back_kernel, dim3(blocks_for(V * H)), dim3(256), 0, c->stream, c->d1, c->a1, V * H);
+    // layer 1
+    if (int r = wgrad(c, x, kInC, c->d1, H, H, c->g + c->off_w[0])) return r;
+    hipLaunchKernelGGL(bias_grad_kernel, dim3(H), dim3(256), 0, c->stream, c->d1, H, H, V, c->g + c->off_b[0]);
+    CHK(hipGetLastError());
+    if (loss_out) {
+        std::vector<float> lp(c->loss_blocks);
+        CHK(hipMemcpyAsync(lp.data(), c->loss_part, lp.size() * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+        CHK(hipStreamSynchronize(c->stream));
+        double s = 0.0;
+        for (float f : lp) s += f;
+        *loss_out = s / (kOutC * (double)V);
+    }
+    return LBM_OK;
+}
+
+int upload_x(lbm_closure* c, const double* x, __bf16* dst) {
+    std::vector<__bf16> h(c->V * 8);
+    for (long i = 0; i < c->V; ++i)
+        for (int m = 0; m < 8; ++m) h[i * 8 + m] = (__bf16)(float)(x[m * c->V + i] * c->h_in_scale[m]);
+    CHK(hipMemcpyAsync(dst, h.data(), h.size() * sizeof(__bf16), hipMemcpyHostToDevice, c->stream));
+    CHK(hipStreamSynchronize(c->stream));
+    return LBM_OK;
+}
+int upload_y(lbm_closure* c, const double* y, float* dst) {
+    std::vector<float> h(c->V * kOutC);
+    for (int m = 0; m < kOutC; ++m)
+        for (long i = 0; i < c->V; ++i) h[m * c->V + i] = (float)(y[m * c->V + i] * c->h_out_scale[m]);
+    CHK(hipMemcpyAsync(dst, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    CHK(hipStreamSynchronize(c->stream));
+    return LBM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lbm_closure_create(int32_t nx, int32_t ny, int32_t nz, int32_t hidden, uint64_t seed, lbm_closure_handle* out) {
+    if (!out) return fail(LBM_ERR_INVALID, "null argument");
+    *out = nullptr;
+    if (hidden != 32) return fail(LBM_ERR_UNSUPPORTED, "the closure kernels are built for 32 hidden channels");
+    if (nx < 1 || ny < 1 || nz < 1 || (long)nx * ny * nz % 16 != 0 || nx % 16 != 0)
+        return fail(LBM_ERR_INVALID, "closure grid must have nx a multiple of 16");
+    lbm_closure* c = new lbm_closure();
+    auto bail = [&](int r) {
+        lbm_closure_destroy(c);
+        return r;
+    };
+    c->nx = nx;
+    c->ny = ny;
+    c->nz = nz;
+    c->hidden = hidden;
+    c->V = (long)nx * ny * nz;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+        return bail(fail(LBM_ERR_HIP, "hipStreamCreate failed"));
+    const int H = hidden;
+    c->cin[0] = kInC, c->cout[0] = H;
+    c->cin[1] = H, c->cout[1] = H;
+    c->cin[2] = H, c->cout[2] = kOutC;
+    long o = 0;
+    for (int l = 0; l < 3; ++l) {
+        c->off_w[l] = o;
+        o += 27L * c->cin[l] * c->cout[l];
+        c->off_b[l] = o;
+        o += c->cout[l];
+    }
+    c->nparams = o;
+    for (int l = 0; l < 3; ++l) {
+        c->kch[l] = c->cin[l] == 8 ? 7 : 27;
+        c->nt[l] = (c->cout[l] + 15) / 16;
+    }
+    // input-gradient convolutions: layer 3 (gradient has 8 padded channels in, H out), layer 2 (H in, H out)
+    c->kcht[1] = 7;
+    c->ntt[1] = (H + 15) / 16;
+    c->kcht[0] = 27;
+    c->ntt[0] = (H + 15) / 16;
+    int r;
+    if ((r = cl_alloc(c, &c->w, c->nparams)) || (r = cl_alloc(c, &c->g, c->nparams)) ||
+        (r = cl_alloc(c, &c->m, c->nparams)) || (r = cl_alloc(c, &c->v, c->nparams)))
+        return bail(r);
+    for (int l = 0; l < 3; ++l)
+        if ((r = cl_alloc(c, &c->pk[l], (size_t)c->kch[l] * c->nt[l] * 64))) return bail(r);
+    for (int l = 0; l < 2; ++l)
+        if ((r = cl_alloc(c, &c->pkt[l], (size_t)c->kcht[l] * c->ntt[l] * 64))) return bail(r);
+    const size_t V = c->V;
+    if ((r = cl_alloc(c, &c->x, V * 8)) || (r = cl_alloc(c, &c->a1, V * H)) || (r = cl_alloc(c, &c->a2, V * H)) ||
+        (r = cl_alloc(c, &c->d3, V * 8)) || (r = cl_alloc(c, &c->d2, V * H)) || (r = cl_alloc(c, &c->d1, V * H)) ||
+        (r = cl_alloc(c, &c->pred, V * kOutC)))
+        return bail(r);
+    const long nchunks = (c->V + c->chunk - 1) / c->chunk;
+    c->part_elems = nchunks * 864L * 32;
+    if ((r = cl_alloc(c, &c->part, c->part_elems))) return bail(r);
+    c->loss_blocks = 512;
+    if ((r = cl_alloc(c, &c->loss_part, c->loss_blocks))) return bail(r);
+    if ((r = cl_alloc(c, &c->in_scale, 8)) || (r = cl_alloc(c, &c->out_scale, 8))) return bail(r);
+    // He-normal initial weights from the counter-based stream, zero biases.
+    std::vector<float> w(c->nparams, 0.f);
+    for (int l = 0; l < 3; ++l) {
+        const double sd = std::sqrt(2.0 / (27.0 * c->cin[l])) * (l == 2 ? 0.1 : 1.0);
+        const long n = 27L * c->cin[l] * c->cout[l];
+        for (long i = 0; i < n; ++i) {
+            const double u1 = std::max(1e-12, uniform01(seed, (uint64_t)(c->off_w[l] + i), 5));
+            const double u2 = uniform01(seed, (uint64_t)(c->off_w[l] + i), 6);
+            w[c->off_w[l] + i] = (float)(sd * std::sqrt(-2.0 * std::log(u1)) * std::cos(2.0 * M_PI * u2));
+        }
+    }
+    if (hipMemcpy(c->w, w.data(), w.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
+        return bail(fail(LBM_ERR_HIP, "weight upload failed"));
+    double ones[8] = {1, 1, 1, 1, 1, 1, 1, 1};
+    if ((r = lbm_closure_set_normalization(c, ones, ones))) return bail(r);
+    if ((r = repack(c))) return bail(r);
+    *out = c;
+    return LBM_OK;
+}
+
+int lbm_closure_destroy(lbm_closure_handle c) {
+    if (!c) return LBM_OK;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (void* p : c->allocs) (void)hipFree(p);
+    for (auto* p : c->sx) (void)hipFree(p);
+    for (auto* p : c->sy) (void)hipFree(p);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return LBM_OK;
+}
+
+int lbm_closure_param_count(lbm_closure_handle c, int64_t* n) {
+    if (!c || !n) return fail(LBM_ERR_INVALID, "null argument");
+    *n = c->nparams;
+    return LBM_OK;
+}
+
+int lbm_closure_get_params(lbm_closure_handle c, double* w) {
+    if (!c || !w) return fail(LBM_ERR_INVALID, "null argument");
+    std::vector<float> h(c->nparams);
+    CHK(hipMemcpyAsync(h.data(), c->w, h.size() * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    CHK(hipStreamSynchronize(c->stream));
+    for (long i = 0; i < c->nparams; ++i) w[i] = h[i];
+    return LBM_OK;
+}
+
+int lbm_closure_set_params(lbm_closure_handle c, const double* w) {
+    if (!c || !w) return fail(LBM_ERR_INVALID, "null argument");
+    std::vector<float> h(c->nparams);
+    for (long i = 0; i < c->nparams; ++i) h[i] = (float)w[i];
+    CHK(hipMemcpyAsync(c->w, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    CHK(hipMemsetAsync(c->m, 0, c->nparams * sizeof(float), c->stream));
+    CHK(hipMemsetAsync(c->v, 0, c->nparams * sizeof(float), c->stream));
+    c->step = 0;
+    if (int r = repack(c)) return r;
+    CHK(hipStreamSynchronize(c->stream));
+    return LBM_OK;
+}
+
+int lbm_closure_set_normalization(lbm_closure_handle c, const double* in_scale, const double* out_scale) {
+    if (!c || !in_scale || !out_scale) return fail(LBM_ERR_INVALID, "null argument");
+    float fi[8], fo[8] = {1, 1, 1, 1, 1, 1, 1, 1};
+    for (int m = 0; m < 8; ++m) fi[m] = (float)(c->h_in_scale[m] = in_scale[m]);
+    for (int m = 0; m < 5; ++m) fo[m] = (float)(c->h_out_scale[m] = out_scale[m]);
+    CHK(hipMemcpyAsync(c->in_scale, fi, sizeof(fi), hipMemcpyHostToDevice, c->stream));
+    CHK(hipMemcpyAsync(c->out_scale, fo, sizeof(fo), hipMemcpyHostToDevice, c->stream));
+    CHK(hipStreamSynchronize(c->stream));
+    return LBM_OK;
+}
+
+int lbm_closure_add_sample(lbm_closure_handle c, const double* x, const double* y, int32_t* index) {
+    if (!c || !x || !y) return fail(LBM_ERR_INVALID, "null argument");
+    __bf16* dx = nullptr;
+    float* dy = nullptr;
+    CHK(hipMalloc(&dx, c->V * 8 * sizeof(__bf16)));
+    if (hipMalloc(&dy, c->V * kOutC * sizeof(float)) != hipSuccess) {
+        (void)hipFree(dx);
+        return fail(LBM_ERR_HIP, "hipMalloc failed");
+    }
+    c->sx.push_back(dx);
+    c->sy.push_back(dy);
+    if (int r = upload_x(c, x, dx)) return r;
+    if (int r = upload_y(c, y, dy)) return r;
+    if (index) *index = (int32_t)c->sx.size() - 1;
+    return LBM_OK;
+}
+
+int lbm_closure_clear_samples(lbm_closure_handle c) {
+    if (!c) return fail(LBM_ERR_INVALID, "null argument");
+    CHK(hipStreamSynchronize(c->stream));
+    for (auto* p : c->sx) (void)hipFree(p);
+    for (auto* p : c->sy) (void)hipFree(p);
+    c->sx.clear();
+    c->sy.clear();
+    return LBM_OK;
+}
+
+int lbm_closure_train(lbm_closure_handle c, int32_t iters, double lr, uint64_t seed, double* mean_loss) {
+    if (!c || iters < 1) return fail(LBM_ERR_INVALID, "bad argument");
+    if (c->sx.empty()) return fail(LBM_ERR_STATE, "no training samples");
+    double acc = 0.0;
+    const int ns = (int)c->sx.size();
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    for (int it = 0; it < iters; ++it) {
+        const int s = (int)(uniform01(seed, (uint64_t)(c->step), 11) * ns) % ns;
+        double loss = 0.0;
+        const bool want = mean_loss != nullptr;
+        if (int r = loss_and_grad(c, c->sx[s], c->sy[s], want ? &loss : nullptr)) return r;
+        acc += loss;
+        ++c->step;
+        const float c1 = 1.f - std::pow(b1, (float)c->step), c2 = 1.f - std::pow(b2, (float)c->step);
+        hipLaunchKernelGGL(adam_kernel, dim3(blocks_for(c->nparams)), dim3(256), 0, c->stream, c->w, c->g, c->m, c->v,
+                           c->nparams, (float)lr, b1, b2, eps, c1, c2);
+        if (int r = repack(c)) return r;
+    }
+    CHK(hipStreamSynchronize(c->stream));
+    if (mean_loss) *mean_loss = acc / iters;
+    return LBM_OK;
+}
+
+int lbm_closure_loss_and_grad(lbm_closure_handle c, const double* x, const double* y, double* loss, double* grad) {
+    if (!c || !x || !y || !loss) return fail(LBM_ERR_INVALID, "null argument");
+    float* dy = nullptr;
+    CHK(hipMalloc(&dy, c->V * kOutC * sizeof(float)));
+    int r = upload_x(c, x, c->x);
+    if (!r) r = upload_y(c, y, dy);
+    if (!r) r = loss_and_grad(c, c->x, dy, loss);
+    if (!r && grad) {
+        std::vector<float> h(c->nparams);
+        if (hipMemcpyAsync(h.data(), c->g, h.size() * sizeof(float), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+            hipStreamSynchronize(c->stream) != hipSuccess)
+            r = fail(LBM_ERR_HIP, "gradient download failed");
+        for (long i = 0; i < c->nparams && !r; ++i) grad[i] = h[i];
+    }
+    (void)hipFree(dy);
+    return r;
+}
+
+int lbm_closure_predict(lbm_closure_handle c, const double* x, double* y) {
+    if (!c || !x || !y) return fail(LBM_ERR_INVALID, "null argument");
+    if (int r = upload_x(c, x, c->x)) return r;
+    if (int r = forward(c, c->x)) return r;
+    std::vector<float> h(c->V * kOutC);
+    CHK(hipMemcpyAsync(h.data(), c->pred, h.size() * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    CHK(hipStreamSynchronize(c->stream));
+    for (int m = 0; m < kOutC; ++m)
+        for (long i = 0; i < c->V; ++i) y[m * c->V + i] = h[m * c->V + i] / c->h_out_scale[m];
+    return LBM_OK;
+}
+
+int lbm_closure_apply(lbm_closure_handle c, lbm_handle sim, double alpha) {
+    if (!c || !sim) return fail(LBM_ERR_INVALID, "null argument");
+    if (sim->fp64) return fail(LBM_ERR_UNSUPPORTED, "the closure corrects FP32 simulations");
+    if (sim->cfg.nx != c->nx || sim->cfg.ny != c->ny || sim->cfg.nz != c->nz)
+        return fail(LBM_ERR_INVALID, "closure and simulation grids differ");
+    CHK(hipStreamSynchronize(sim->stream));
+    hipLaunchKernelGGL(gather_input_kernel, dim3(blocks_for(c->V)), dim3(256), 0, c->stream, sim->d32, c->V,
+                       c->in_scale, c->x);
+    if (int r = forward(c, c->x)) return r;
+    hipLaunchKernelGGL(apply_kernel, dim3(blocks_for(c->V)), dim3(256), 0, c->stream, sim->d32, c->V, c->pred,
+                       c->out_scale, (float)alpha);
+    CHK(hipGetLastError());
+    CHK(hipStreamSynchronize(c->stream));
+    return LBM_OK;
+}
+
+int lbm_closure_time_forward(lbm_closure_handle c, int32_t n, double* ms) {
+    if (!c || !ms || n < 1) return fail(LBM_ERR_INVALID, "bad argument");
+    hipEvent_t a, b;
+    CHK(hipEventCreate(&a));
+    CHK(hipEventCreate(&b));
+    CHK(hipEventRecord(a, c->stream));
+    for (int i = 0; i < n; ++i)
+        if (int r = forward(c, c->x)) return r;
+    CHK(hipEventRecord(b, c->stream));
+    CHK(hipEventSynchronize(b));
+    float f = 0.f;
+    CHK(hipEventElapsedTime(&f, a, b));
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    *ms = f / n;
+    return LBM_OK;
+}
+
+int lbm_debug_mfma_bf16(const float* a, const float* b, float* d) {
+    if (!a || !b || !d) return fail(LBM_ERR_INVALID, "null argument");
+    float *da = nullptr, *db = nullptr, *dd = nullptr;
+    CHK(hipMalloc(&da, 16 * 32 * sizeof(float)));
+    CHK(hipMalloc(&db, 32 * 16 * sizeof(float)));
+    CHK(hipMalloc(&dd, 16 * 16 * sizeof(float)));
+    CHK(hipMemcpy(da, a, 16 * 32 * sizeof(float), hipMemcpyHostToDevice));
+    CHK(hipMemcpy(db, b, 32 * 16 * sizeof(float), hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(mfma_check_kernel, dim3(1), dim3(64), 0, 0, da, db, dd);
+    CHK(hipGetLastError());
+    CHK(hipMemcpy(d, dd, 16 * 16 * sizeof(float), hipMemcpyDeviceToHost));
+    (void)hipFree(da);
+    (void)hipFree(db);
+    (void)hipFree(dd);
+    return LBM_OK;
+}
+
+}  // extern "C"

@@ -64,6 +64,21 @@ SIGNATURES = {
     "lbm_batch_set_tile": (C.c_int, [C.c_void_p, C.c_int32]),
     "lbm_batch_get_tile": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32)]),
     "lbm_batch_autotune": (C.c_int, [C.c_void_p, C.c_int32, _D, C.c_int32, C.POINTER(C.c_int32)]),
+    "lbm_closure_create": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_uint64,
+                                     C.POINTER(C.c_void_p)]),
+    "lbm_closure_destroy": (C.c_int, [C.c_void_p]),
+    "lbm_closure_param_count": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64)]),
+    "lbm_closure_get_params": (C.c_int, [C.c_void_p, _D]),
+    "lbm_closure_set_params": (C.c_int, [C.c_void_p, _D]),
+    "lbm_closure_set_normalization": (C.c_int, [C.c_void_p, _D, _D]),
+    "lbm_closure_add_sample": (C.c_int, [C.c_void_p, _D, _D, C.POINTER(C.c_int32)]),
+    "lbm_closure_clear_samples": (C.c_int, [C.c_void_p]),
+    "lbm_closure_train": (C.c_int, [C.c_void_p, C.c_int32, C.c_double, C.c_uint64, _D]),
+    "lbm_closure_loss_and_grad": (C.c_int, [C.c_void_p, _D, _D, _D, _D]),
+    "lbm_closure_predict": (C.c_int, [C.c_void_p, _D, _D]),
+    "lbm_closure_apply": (C.c_int, [C.c_void_p, C.c_void_p, C.c_double]),
+    "lbm_closure_time_forward": (C.c_int, [C.c_void_p, C.c_int32, _D]),
+    "lbm_debug_mfma_bf16": (C.c_int, [C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float)]),
 }
 
 

@@ -15,10 +15,12 @@
 // without FMA contraction (-ffp-contract=off, as the reference's own CPU build
 // compiles, CMakeLists.txt:50-62) that order fixes the result bits.
 //
-// Parity pins (see tests/): every function here is checked against the
-// reference compiled from its own sources (oracle/_ref, tests/test_oracle_ref.py)
-// and against the closed forms of the reference's unit tests
-// (tests/test_oracle_kat.py).
+// Parity pins (see tests/): every function here is checked bit for bit
+// against the reference's own CPU solver compiled from its sources, through
+// the committed fixtures tests/golden/*.npz and a live run of oracle/_ref
+// where the reference is present (tests/test_oracle_golden.py), and against
+// the closed forms the reference's integration tests use (Poiseuille profile,
+// equilibrium scalar order, conservation; tests/test_invariants.py).
 
 #include <cmath>
 #include <cstdint>

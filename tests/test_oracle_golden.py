@@ -4,8 +4,8 @@ Pins: every committed fixture under tests/golden/ was produced by the
 reference's CPU solver (src/lbm_solver.tpp, src/qtensor_solver.tpp,
 src/analysis_fields.h) compiled from its sources (oracle/build_ref.sh,
 tests/golden/make_golden.py). The oracle must reproduce each one bit for bit
-(rho, u, Q, F, populations) and the free energy exactly (single-threaded
-summation order).
+(rho, u, Q, F, populations), the free energy and the SimIO::Log sums
+exactly (single-threaded summation order).
 """
 import os
 
@@ -31,6 +31,8 @@ def test_oracle_matches_reference_bitwise(name, oracle_lib):
     if "f" in g:
         np.testing.assert_array_equal(o.populations(), g["f"])
     assert o.free_energy() == float(g["free_energy"])
+    # SimIO::Log sums (src/sim_io.cc:78-165), single-threaded order
+    np.testing.assert_array_equal(np.array(list(o.log_sums().values())), g["log"])
 
 
 def test_fixture_set_is_complete():
@@ -73,3 +75,25 @@ def test_oracle_thread_count_independent(oracle_lib):
         run_golden_on(s, g)
     np.testing.assert_array_equal(a.state()["q"], b.state()["q"])
     np.testing.assert_array_equal(a.populations(), b.populations())
+
+
+def test_order_director_closed_form(oracle_lib):
+    """Q = S (n n - I/3) has largest eigenvalue 2S/3 with eigenvector n. The
+    reference reports that eigenvalue as the order (QtensorToOrderDirector,
+    src/analysis_fields.cc:59-93; its relaxation test expects -B/(3C) =
+    2 S_eq / 3, tests/integration/test_qtensor_relaxation.cc:154-157) and
+    +-n as the director."""
+    from lbman3d_amd import abi
+    dims = (6, 5, 4)
+    V = 6 * 5 * 4
+    rng = np.random.default_rng(12)
+    n = rng.normal(size=(V, 3))
+    n /= np.linalg.norm(n, axis=1, keepdims=True)
+    S = rng.uniform(0.1, 0.9, V)
+    q = np.stack([S * (n[:, 0] ** 2 - 1 / 3), S * n[:, 0] * n[:, 1], S * n[:, 0] * n[:, 2],
+                  S * (n[:, 1] ** 2 - 1 / 3), S * n[:, 1] * n[:, 2]])
+    o = oracle_lib.Oracle(abi.make_config(*dims))
+    o.set_state(q=q)
+    order, director = o.order_director()
+    np.testing.assert_allclose(order, 2 * S / 3, rtol=1e-10)
+    np.testing.assert_allclose(np.abs(np.sum(director * n, axis=1)), 1.0, rtol=1e-10)
