@@ -67,5 +67,21 @@ def build(jobs: int = 4) -> str:
     return LIB
 
 
+HOST_CHECK_SRC = os.path.join(HERE, "..", "tests", "cpp", "host_api_check.cpp")
+HOST_CHECK_BIN = os.path.join(HERE, "..", "tests", "cpp", "host_api_check")
+
+
+def build_host_check() -> str:
+    """Compile the C++ host-interface check (tests/cpp/host_api_check.cpp)
+    against include/lbman3d.hpp and the in-tree library."""
+    cmd = ["g++", "-std=c++20", "-O1", "-Wall", "-I", os.path.join(HERE, "..", "include"), HOST_CHECK_SRC,
+           "-L", HERE, "-llbman3d", "-Wl,-rpath,$ORIGIN/../../lbman3d_amd", "-o", HOST_CHECK_BIN]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"host check build failed:\n{' '.join(cmd)}\n{r.stderr}")
+    return HOST_CHECK_BIN
+
+
 if __name__ == "__main__":
     print(build())
+    print(build_host_check())
