@@ -816,6 +816,10 @@ constexpr TileShape kTiles[] = {
     {0, 32, 8, 32, "32x8x32"},        {0, 32, 8, 64, "32x8x64"},        {0, 64, 4, 32, "64x4x32"},
     {1, 64, 4, 64, "pf:64x4x64"},     {1, 32, 8, 64, "pf:32x8x64"},     {1, 64, 4, 32, "pf:64x4x32"},
     {1, 32, 8, 32, "pf:32x8x32"},     {2, 64, 4, 64, "pf-nt:64x4x64"},  {2, 32, 8, 64, "pf-nt:32x8x64"},
+    // 512-thread tiles: the ring of halo columns whose Q update is recomputed
+    // is a smaller share of the interior (64x8: 148 of 512, 32x16: 100 of 512)
+    {1, 64, 8, 64, "pf:64x8x64"},     {1, 32, 16, 64, "pf:32x16x64"},   {1, 64, 8, 32, "pf:64x8x32"},
+    {1, 32, 16, 32, "pf:32x16x32"},
 };
 constexpr int kNumTiles = sizeof(kTiles) / sizeof(kTiles[0]);
 
@@ -863,7 +867,11 @@ void launch_fused_step_f32(const Fields<float>& f, const BCDesc<float>& bc, cons
         case 5: launch_tile<1, 64, 4, 32>(f, bc, k, pull, aux, s); break;
         case 6: launch_tile<1, 32, 8, 32>(f, bc, k, pull, aux, s); break;
         case 7: launch_tile<2, 64, 4, 64>(f, bc, k, pull, aux, s); break;
-        default: launch_tile<2, 32, 8, 64>(f, bc, k, pull, aux, s); break;
+        case 8: launch_tile<2, 32, 8, 64>(f, bc, k, pull, aux, s); break;
+        case 9: launch_tile<1, 64, 8, 64>(f, bc, k, pull, aux, s); break;
+        case 10: launch_tile<1, 32, 16, 64>(f, bc, k, pull, aux, s); break;
+        case 11: launch_tile<1, 64, 8, 32>(f, bc, k, pull, aux, s); break;
+        default: launch_tile<1, 32, 16, 32>(f, bc, k, pull, aux, s); break;
     }
 }
 
