@@ -6,9 +6,10 @@ zeta uniform in [0.5, 2] x 0.023, flow alignment lambda uniform in [0.3, 0.7],
 centred advection, each simulation seeded with its own random director field
 (S0 = 0.5 +- 0.01). The batch is stepped by the persistent batched kernel from
 a replayed hipGraph; the same 64 simulations are then stepped one after
-another as single handles (each on its own autotuned fused schedule), and the
-aggregate MLUPS of both are reported together with the number of simulations
-that diverged (non-finite mass or momentum) in the batched run.
+another as single handles (each on its own autotuned fused schedule), and
+once more one after another on the FP64 plain path. The aggregate MLUPS of
+all three are reported together with the number of simulations that
+diverged (non-finite mass or momentum) in the batched run.
 
 usage: python -m lbman3d_amd.batchbench [--steps 5000] [--sims 64]
 """
@@ -71,10 +72,21 @@ def main():
             s.step(2)
             seq_ms += s.time_steps(a.seq_steps)
     seq_mlups = cells * a.seq_steps / (seq_ms * 1e-3) / 1e6
+
+    # the FP64 plain path (reference-parity kernels), same simulations one by one
+    f64_ms = 0.0
+    for c, q in zip(cfgs, qs):
+        with DeviceSolver(abi.config_copy(c, precision=abi.FP64)) as s:
+            s.upload_state(q=q)
+            s.step(2)
+            f64_ms += s.time_steps(a.seq_steps)
+    f64_mlups = cells * a.seq_steps / (f64_ms * 1e-3) / 1e6
     print(json.dumps({"workload": "batched sweep", "sims": len(cfgs), "cells": cells, "steps": a.steps,
                       "batch_ms_per_step": ms / a.steps, "batch_mlups": batch_mlups, "batch_wall_s": wall,
                       "batch_tile_ms": tune, "sequential_ms_per_step": seq_ms / a.seq_steps,
                       "sequential_mlups": seq_mlups, "speedup": batch_mlups / seq_mlups,
+                      "sequential_fp64_ms_per_step": f64_ms / a.seq_steps, "sequential_fp64_mlups": f64_mlups,
+                      "speedup_vs_fp64_plain": batch_mlups / f64_mlups,
                       "diverged": diverged, "umax": [d["umax"] for d in diags],
                       "sims_meta": meta}), flush=True)
 

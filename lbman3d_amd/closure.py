@@ -253,6 +253,17 @@ def run(n_fine=128, samples=48, heldout=8, spacing=20, warmup=1500, iters=1500, 
             start, targets = fine_targets(fine, n_fine, roll_steps, roll_every)
             err_no = rollout(coarse, None, start, targets, roll_every)
             err_cl = rollout(coarse, net, start, targets, roll_every)
+            # time to solution per coarse step (= two fine steps)
+            fine_ms = fine.time_steps(100) / 100 * 2
+            coarse_ms = coarse.time_steps(100) / 100
+            reps = 50
+            coarse.synchronize()
+            t2 = time.perf_counter()
+            for _ in range(reps):
+                coarse.step(1)
+                net.apply(coarse, 1.0)
+            coarse.synchronize()
+            coarse_cl_ms = (time.perf_counter() - t2) * 1e3 / reps
 
     report.update({
         "samples": samples, "heldout": heldout, "spacing": spacing, "warmup": warmup, "iters": iters, "lr": lr,
@@ -261,6 +272,9 @@ def run(n_fine=128, samples=48, heldout=8, spacing=20, warmup=1500, iters=1500, 
         "onestep_gain": 1.0 - float(np.mean(one_cl)) / float(np.mean(one_no)),
         "rollout_every": roll_every, "rollout_relL2_no_closure": err_no, "rollout_relL2_closure": err_cl,
         "forward_ms": ms_fwd, "forward_tflops": FLOP_PER_CELL * nc ** 3 / (ms_fwd * 1e-3) / 1e12,
+        "fine_fp64_ms_per_2_steps": fine_ms, "coarse_fp32_ms_per_step": coarse_ms,
+        "coarse_with_closure_wall_ms_per_step": coarse_cl_ms,
+        "speedup_vs_fine_fp64": fine_ms / coarse_cl_ms,
     })
     return report
 
