@@ -269,7 +269,10 @@ int lbm_closure_loss_and_grad(lbm_closure_handle c, const double* x, const doubl
                               double* grad);
 /* y = network(x) in physical units (the output scale undone). */
 int lbm_closure_predict(lbm_closure_handle c, const double* x, double* y);
-/* Adds alpha * network(Q, u) to the Q of an FP32 simulation of the same grid. */
+/* Adds alpha * network(Q, u) to the Q of an FP32 simulation of the same grid.
+ * Asynchronous: enqueued on the simulation's stream after the closure's
+ * pending work (later closure calls wait for it); calls on one closure must
+ * not overlap from several threads. */
 int lbm_closure_apply(lbm_closure_handle c, lbm_handle sim, double alpha);
 /* Mean time of one forward pass over the whole grid (ms), n passes. */
 int lbm_closure_time_forward(lbm_closure_handle c, int32_t n, double* ms);

@@ -53,111 +53,157 @@ __device__ __forceinline__ int nbr(int x, int y, int z, int off, int nx, int ny,
 }
 
 // ---- forward convolution (also the input-gradient convolution) -------------
-// One wavefront: 16 consecutive cells x (16 * NT) output channels. Cin is 8
-// (four stencil offsets per 32-wide K step) or 32 (one offset per K step).
+// One wavefront: MT row tiles of 16 consecutive cells x (16 * NT) output
+// channels; each weight fragment is loaded once per K step and used for all
+// MT row tiles. Cin is 8 (four stencil offsets per 32-wide K step) or 32 (one
+// offset per K step).
 // EPI: 0 = store BF16 with bias + ReLU, 1 = store BF16 linear (no bias),
 // 2 = store FP32 channels-first [co][cell] with bias (final layer).
-template <int CIN, int NT, int EPI>
+template <int CIN, int NT, int MT, int EPI>
 __global__ void __launch_bounds__(256)
 conv_fwd_kernel(const __bf16* __restrict__ in, const bf16x8* __restrict__ wpk, const float* __restrict__ bias,
                 void* __restrict__ out, int cout, int nx, int ny, int nz) {
     const int lane = threadIdx.x & 63;
     const long wave = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const long V = (long)nx * ny * nz;
-    const long c0 = wave * 16;
+    const long c0 = wave * 16 * MT;
     if (c0 >= V) return;
     const int m = lane & 15, kb = lane >> 4;
-    const long cm = c0 + m;
-    const int x = (int)(cm % nx), y = (int)((cm / nx) % ny), z = (int)(cm / ((long)nx * ny));
+    int xs[MT], ys[MT], zs[MT];
+    bool ok[MT];
+#pragma unroll
+    for (int t = 0; t < MT; ++t) {
+        const long cm = c0 + 16 * t + m;
+        ok[t] = cm < V;
+        xs[t] = (int)(cm % nx);
+        ys[t] = (int)((cm / nx) % ny);
+        zs[t] = (int)(cm / ((long)nx * ny));
+    }
     constexpr int KCH = CIN == 8 ? 7 : 27;   // K steps of 32 (8-channel input: 28 offsets, last zero)
-    f32x4 acc[NT];
+    f32x4 acc[MT][NT];
 #pragma unroll
-    for (int t = 0; t < NT; ++t) acc[t] = (f32x4){0.f, 0.f, 0.f, 0.f};
+    for (int t = 0; t < MT; ++t)
+#pragma unroll
+        for (int n = 0; n < NT; ++n) acc[t][n] = (f32x4){0.f, 0.f, 0.f, 0.f};
     for (int kc = 0; kc < KCH; ++kc) {
-        bf16x8 a;
-        int off, ch;
-        if (CIN == 8) {
-            off = kc * 4 + kb;
-            ch = 0;
-        } else {
-            off = kc;
-            ch = kb * 8;
-        }
-        if (off < kOffsets) {
-            const int nc = nbr(x, y, z, off, nx, ny, nz);
-            a = *reinterpret_cast<const bf16x8*>(in + (long)nc * CIN + ch);
-        } else {
+        const int off = CIN == 8 ? kc * 4 + kb : kc;
+        const int ch = CIN == 8 ? 0 : kb * 8;
+        bf16x8 b[NT];
 #pragma unroll
-            for (int j = 0; j < 8; ++j) a[j] = to_bf16(0.f);
-        }
+        for (int n = 0; n < NT; ++n) b[n] = wpk[((long)kc * NT + n) * 64 + lane];
 #pragma unroll
-        for (int t = 0; t < NT; ++t) {
-            const bf16x8 b = wpk[((long)kc * NT + t) * 64 + lane];
-            acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[t], 0, 0, 0);
+        for (int t = 0; t < MT; ++t) {
+            bf16x8 a;
+            if (off < kOffsets && ok[t]) {
+                const int nc = nbr(xs[t], ys[t], zs[t], off, nx, ny, nz);
+                a = *reinterpret_cast<const bf16x8*>(in + (long)nc * CIN + ch);
+            } else {
+#pragma unroll
+                for (int j = 0; j < 8; ++j) a[j] = to_bf16(0.f);
+            }
+#pragma unroll
+            for (int n = 0; n < NT; ++n)
+                acc[t][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[n], acc[t][n], 0, 0, 0);
         }
     }
     // D fragment: lane holds rows 4*kb + r (cells), column n = m (channel).
 #pragma unroll
-    for (int t = 0; t < NT; ++t) {
-        const int co = t * 16 + m;
+    for (int n = 0; n < NT; ++n) {
+        const int co = n * 16 + m;
         if (co >= cout) continue;
         const float bco = (EPI == 1) ? 0.f : bias[co];
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const long cell = c0 + 4 * kb + r;
-            float v = acc[t][r] + bco;
-            if (EPI == 0) v = v > 0.f ? v : 0.f;
-            if (EPI == 2) reinterpret_cast<float*>(out)[(long)co * V + cell] = v;
-            else reinterpret_cast<__bf16*>(out)[cell * cout + co] = to_bf16(v);
+        for (int t = 0; t < MT; ++t) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const long cell = c0 + 16 * t + 4 * kb + r;
+                if (cell >= V) continue;
+                float v = acc[t][n][r] + bco;
+                if (EPI == 0) v = v > 0.f ? v : 0.f;
+                if (EPI == 2) reinterpret_cast<float*>(out)[(long)co * V + cell] = v;
+                else reinterpret_cast<__bf16*>(out)[cell * cout + co] = to_bf16(v);
+            }
         }
     }
 }
 
 // ---- weight gradient --------------------------------------------------------
-// dW[k][co] = sum_cell A[cell][k] dOut[cell][co] for one 16 x 16 (k, co) tile
-// per wavefront over a chunk of cells; partial sums land in
-// part[chunk][k][co] and are reduced by wgrad_reduce_kernel.
-template <int CIN>
+// dW[k][co] = sum_cell A[cell][k] dOut[cell][co], k = off * CIN + ci. One
+// wavefront owns KT 16-row k tiles and all NT 16-column co tiles over a chunk
+// of cells, so each dOut fragment serves KT tiles and each gathered input
+// fragment NT tiles. A lane's 8 cells of a 32-cell step are consecutive and
+// lie in one x row (nx is a multiple of 8), so the neighbour row is found
+// once per step. Partial sums land in part[chunk][k][co] and are reduced by
+// wgrad_reduce_kernel.
+template <int CIN, int KT, int NT>
 __global__ void __launch_bounds__(256)
 conv_wgrad_kernel(const __bf16* __restrict__ in, const __bf16* __restrict__ dout, int cout_pad, int cout,
-                  float* __restrict__ part, int ktiles, int ntiles, int chunk, int nx, int ny, int nz) {
+                  float* __restrict__ part, int ktiles, int chunk, int nx, int ny, int nz) {
     const int lane = threadIdx.x & 63;
     const long wave = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const long V = (long)nx * ny * nz;
     const int nchunks = (int)((V + chunk - 1) / chunk);
-    const long tiles = (long)ktiles * ntiles;
-    if (wave >= tiles * nchunks) return;
-    const int ch_id = (int)(wave / tiles);
-    const int tile = (int)(wave % tiles);
-    const int kt = tile / ntiles, nt = tile % ntiles;
+    const int kgroups = (ktiles + KT - 1) / KT;
+    if (wave >= (long)kgroups * nchunks) return;
+    const int ch_id = (int)(wave / kgroups);
+    const int kg = (int)(wave % kgroups);
     const int m = lane & 15, kb = lane >> 4;
-    // A: row = k index kt*16 + m, 8 consecutive cells; B: column co = nt*16 + m.
-    const int k = kt * 16 + m;
-    const int off = k / CIN, ci = k % CIN;
-    const int co = nt * 16 + m;
-    f32x4 acc = (f32x4){0.f, 0.f, 0.f, 0.f};
-    const long cbeg = (long)ch_id * chunk, cend = std::min<long>(cbeg + chunk, V);
-    for (long cb = cbeg; cb < cend; cb += 32) {
-        bf16x8 a, b;
+    int ci[KT], dx[KT], dy[KT], dz[KT];
+    bool valid[KT];
 #pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            const long cell = cb + 8 * kb + j;
-            float av = 0.f, bv = 0.f;
-            if (cell < cend) {
-                const int x = (int)(cell % nx), y = (int)((cell / nx) % ny), z = (int)(cell / ((long)nx * ny));
-                if (off < kOffsets) av = from_bf16(in[(long)nbr(x, y, z, off, nx, ny, nz) * CIN + ci]);
-                if (co < cout) bv = from_bf16(dout[cell * cout_pad + co]);
-            }
-            a[j] = to_bf16(av);
-            b[j] = to_bf16(bv);
-        }
-        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
+    for (int t = 0; t < KT; ++t) {
+        const int kt = kg * KT + t;
+        const int k = kt * 16 + m;
+        const int off = k / CIN;
+        ci[t] = k % CIN;
+        valid[t] = kt < ktiles && off < kOffsets;
+        dz[t] = off / 9 - 1;
+        dy[t] = (off / 3) % 3 - 1;
+        dx[t] = off % 3 - 1;
     }
-    // D: rows k = kt*16 + 4*kb + r, column co.
+    f32x4 acc[KT][NT];
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-        const int kr = kt * 16 + 4 * kb + r;
-        part[((long)ch_id * (ktiles * 16) + kr) * (ntiles * 16) + co] = acc[r];
+    for (int t = 0; t < KT; ++t)
+#pragma unroll
+        for (int n = 0; n < NT; ++n) acc[t][n] = (f32x4){0.f, 0.f, 0.f, 0.f};
+    const long cbeg = (long)ch_id * chunk, cend = cbeg + chunk < V ? cbeg + chunk : V;
+    for (long cb = cbeg; cb < cend; cb += 32) {
+        const long g0 = cb + 8 * kb;
+        const bool gok = g0 < cend;
+        const int x0 = (int)(g0 % nx), y = (int)((g0 / nx) % ny), z = (int)(g0 / ((long)nx * ny));
+        bf16x8 b[NT];
+#pragma unroll
+        for (int n = 0; n < NT; ++n) {
+            const int co = n * 16 + m;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) b[n][j] = (gok && co < cout) ? dout[(g0 + j) * cout_pad + co] : to_bf16(0.f);
+        }
+#pragma unroll
+        for (int t = 0; t < KT; ++t) {
+            bf16x8 a;
+            if (gok && valid[t]) {
+                const long row = ((long)wrap(z + dz[t], nz) * ny + wrap(y + dy[t], ny)) * nx;
+#pragma unroll
+                for (int j = 0; j < 8; ++j) a[j] = in[(row + wrap(x0 + j + dx[t], nx)) * CIN + ci[t]];
+            } else {
+#pragma unroll
+                for (int j = 0; j < 8; ++j) a[j] = to_bf16(0.f);
+            }
+#pragma unroll
+            for (int n = 0; n < NT; ++n) acc[t][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[n], acc[t][n], 0, 0, 0);
+        }
+    }
+    // D: rows k = kt*16 + 4*kb + r, column co = n*16 + m.
+    const int krows = ktiles * 16, ncols = NT * 16;
+#pragma unroll
+    for (int t = 0; t < KT; ++t) {
+        const int kt = kg * KT + t;
+        if (kt >= ktiles) continue;
+#pragma unroll
+        for (int n = 0; n < NT; ++n)
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                part[((long)ch_id * krows + kt * 16 + 4 * kb + r) * ncols + n * 16 + m] = acc[t][n][r];
     }
 }
 
@@ -171,20 +217,25 @@ __global__ void wgrad_reduce_kernel(const float* __restrict__ part, int nchunks,
     grad[i] = s;   // [k][co] = [off][ci][co]
 }
 
-// Bias gradient: sum over cells of dOut[cell][co].
-__global__ void bias_grad_kernel(const __bf16* __restrict__ dout, int cout_pad, int cout, long V,
-                                 float* __restrict__ grad) {
+// Bias gradient, first pass: per-block sums of dOut[cell][co] over a range
+// of cells (cpad, the stored channel count, divides 256); the per-block
+// partials are then added by wgrad_reduce_kernel.
+__global__ void __launch_bounds__(256)
+bias_partial_kernel(const __bf16* __restrict__ dout, int cpad, long V, float* __restrict__ part) {
     __shared__ float red[256];
-    const int co = blockIdx.x;
+    const int t = threadIdx.x;
+    const int co = t % cpad, sub = t / cpad, step = 256 / cpad;
+    const long per = (V + gridDim.x - 1) / gridDim.x;
+    const long beg = (long)blockIdx.x * per, end = beg + per < V ? beg + per : V;
     float s = 0.f;
-    for (long c = threadIdx.x; c < V; c += blockDim.x) s += from_bf16(dout[c * cout_pad + co]);
-    red[threadIdx.x] = s;
+    for (long c = beg + sub; c < end; c += step) s += from_bf16(dout[c * cpad + co]);
+    red[t] = s;
     __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-        __syncthreads();
+    if (t < cpad) {
+        float a = 0.f;
+        for (int j = t; j < 256; j += cpad) a += red[j];
+        part[(long)blockIdx.x * cpad + t] = a;
     }
-    if (threadIdx.x == 0) grad[co] = red[0];
 }
 
 // Mean-squared loss over V cells x 5 channels: dL/dpred = 2 (pred - y) / (5 V),
@@ -326,6 +377,7 @@ struct lbm_closure {
     int loss_blocks = 0;
     float* in_scale = nullptr;
     float* out_scale = nullptr;
+    hipEvent_t ev_before = nullptr, ev_after = nullptr;   // order apply against the closure's own stream
     double h_in_scale[8] = {1, 1, 1, 1, 1, 1, 1, 1}, h_out_scale[5] = {1, 1, 1, 1, 1};
     std::vector<__bf16*> sx;   // stored samples: inputs
     std::vector<float*> sy;    // stored samples: targets [5][cell] normalised
@@ -369,32 +421,42 @@ int repack(lbm_closure* c) {
     return LBM_OK;
 }
 
-// Forward pass of the input x (BF16 [cell][8]) into c->pred.
-int forward(lbm_closure* c, const __bf16* x) {
-    const unsigned grid = blocks_for(c->V / 16 * 64);
+constexpr int kMT = 4;   // row tiles of 16 cells per wavefront in the convolutions
+unsigned conv_grid(long V) { return blocks_for((V + 16 * kMT - 1) / (16 * kMT) * 64); }
+
+// Forward pass of the input x (BF16 [cell][8]) into c->pred, on stream s.
+int forward(lbm_closure* c, const __bf16* x, hipStream_t s) {
+    const unsigned grid = conv_grid(c->V);
     const int H = c->hidden;
-    hipLaunchKernelGGL((conv_fwd_kernel<8, 2, 0>), dim3(grid), dim3(256), 0, c->stream, x, c->pk[0],
+    hipLaunchKernelGGL((conv_fwd_kernel<8, 2, kMT, 0>), dim3(grid), dim3(256), 0, s, x, c->pk[0],
                        c->w + c->off_b[0], (void*)c->a1, H, c->nx, c->ny, c->nz);
-    hipLaunchKernelGGL((conv_fwd_kernel<32, 2, 0>), dim3(grid), dim3(256), 0, c->stream, c->a1, c->pk[1],
+    hipLaunchKernelGGL((conv_fwd_kernel<32, 2, kMT, 0>), dim3(grid), dim3(256), 0, s, c->a1, c->pk[1],
                        c->w + c->off_b[1], (void*)c->a2, H, c->nx, c->ny, c->nz);
-    hipLaunchKernelGGL((conv_fwd_kernel<32, 1, 2>), dim3(grid), dim3(256), 0, c->stream, c->a2, c->pk[2],
+    hipLaunchKernelGGL((conv_fwd_kernel<32, 1, kMT, 2>), dim3(grid), dim3(256), 0, s, c->a2, c->pk[2],
                        c->w + c->off_b[2], (void*)c->pred, kOutC, c->nx, c->ny, c->nz);
     CHK(hipGetLastError());
     return LBM_OK;
 }
 
 int wgrad(lbm_closure* c, const __bf16* in, int cin, const __bf16* dout, int cout_pad, int cout, float* gw) {
+    constexpr int KT = 2;
     const int krows = ((cin == 8 ? 28 : 27) * cin + 15) / 16 * 16;
     const int ktiles = krows / 16, ntiles = (cout + 15) / 16;
     const int nchunks = (int)((c->V + c->chunk - 1) / c->chunk);
-    const long waves = (long)ktiles * ntiles * nchunks;
+    const long waves = (long)((ktiles + KT - 1) / KT) * nchunks;
     if ((long)nchunks * krows * ntiles * 16 > c->part_elems) return fail(LBM_ERR_STATE, "wgrad scratch too small");
-    if (cin == 8)
-        hipLaunchKernelGGL((conv_wgrad_kernel<8>), dim3(blocks_for(waves * 64)), dim3(256), 0, c->stream, in, dout,
-                           cout_pad, cout, c->part, ktiles, ntiles, c->chunk, c->nx, c->ny, c->nz);
+    const dim3 grid(blocks_for(waves * 64));
+    if (cin == 8 && ntiles == 2)
+        hipLaunchKernelGGL((conv_wgrad_kernel<8, KT, 2>), grid, dim3(256), 0, c->stream, in, dout, cout_pad, cout,
+                           c->part, ktiles, c->chunk, c->nx, c->ny, c->nz);
+    else if (cin == 32 && ntiles == 2)
+        hipLaunchKernelGGL((conv_wgrad_kernel<32, KT, 2>), grid, dim3(256), 0, c->stream, in, dout, cout_pad, cout,
+                           c->part, ktiles, c->chunk, c->nx, c->ny, c->nz);
+    else if (cin == 32 && ntiles == 1)
+        hipLaunchKernelGGL((conv_wgrad_kernel<32, KT, 1>), grid, dim3(256), 0, c->stream, in, dout, cout_pad, cout,
+                           c->part, ktiles, c->chunk, c->nx, c->ny, c->nz);
     else
-        hipLaunchKernelGGL((conv_wgrad_kernel<32>), dim3(blocks_for(waves * 64)), dim3(256), 0, c->stream, in, dout,
-                           cout_pad, cout, c->part, ktiles, ntiles, c->chunk, c->nx, c->ny, c->nz);
+        return fail(LBM_ERR_UNSUPPORTED, "closure layer shape");
     const int kvalid = 27 * cin;
     hipLaunchKernelGGL(wgrad_reduce_kernel, dim3(blocks_for((long)kvalid * cout)), dim3(256), 0, c->stream, c->part,
                        nchunks, krows, ntiles * 16, kvalid, cout, gw);
@@ -402,31 +464,39 @@ int wgrad(lbm_closure* c, const __bf16* in, int cin, const __bf16* dout, int cou
     return LBM_OK;
 }
 
+int bias_grad(lbm_closure* c, const __bf16* dout, int cpad, int cout, float* gb) {
+    constexpr int kBlocks = 512;
+    hipLaunchKernelGGL(bias_partial_kernel, dim3(kBlocks), dim3(256), 0, c->stream, dout, cpad, c->V, c->part);
+    hipLaunchKernelGGL(wgrad_reduce_kernel, dim3(1), dim3(256), 0, c->stream, c->part, kBlocks, 1, cpad, 1, cout,
+                       gb);
+    CHK(hipGetLastError());
+    return LBM_OK;
+}
+
 // Forward + backward for one sample; gradient into c->g, loss (sum of
 // squared errors / (5V)) returned through loss_out (synchronises).
 int loss_and_grad(lbm_closure* c, const __bf16* x, const float* y, double* loss_out) {
-    if (int r = forward(c, x)) return r;
+    if (int r = forward(c, x, c->stream)) return r;
     const int H = c->hidden;
     const long V = c->V;
     hipLaunchKernelGGL(loss_grad_kernel, dim3(c->loss_blocks), dim3(256), 0, c->stream, c->pred, y, V, c->d3,
                        c->loss_part);
     // layer 3: weights, bias, input gradient (8-channel padded gradient in, H out)
     if (int r = wgrad(c, c->a2, H, c->d3, 8, kOutC, c->g + c->off_w[2])) return r;
-    hipLaunchKernelGGL(bias_grad_kernel, dim3(kOutC), dim3(256), 0, c->stream, c->d3, 8, kOutC, V,
-                       c->g + c->off_b[2]);
-    const unsigned grid = blocks_for(V / 16 * 64);
-    hipLaunchKernelGGL((conv_fwd_kernel<8, 2, 1>), dim3(grid), dim3(256), 0, c->stream, c->d3, c->pkt[1], nullptr,
+    if (int r = bias_grad(c, c->d3, 8, kOutC, c->g + c->off_b[2])) return r;
+    const unsigned grid = conv_grid(V);
+    hipLaunchKernelGGL((conv_fwd_kernel<8, 2, kMT, 1>), dim3(grid), dim3(256), 0, c->stream, c->d3, c->pkt[1], nullptr,
                        (void*)c->d2, H, c->nx, c->ny, c->nz);
     hipLaunchKernelGGL(relu_back_kernel, dim3(blocks_for(V * H)), dim3(256), 0, c->stream, c->d2, c->a2, V * H);
     // layer 2
     if (int r = wgrad(c, c->a1, H, c->d2, H, H, c->g + c->off_w[1])) return r;
-    hipLaunchKernelGGL(bias_grad_kernel, dim3(H), dim3(256), 0, c->stream, c->d2, H, H, V, c->g + c->off_b[1]);
-    hipLaunchKernelGGL((conv_fwd_kernel<32, 2, 1>), dim3(grid), dim3(256), 0, c->stream, c->d2, c->pkt[0], nullptr,
+    if (int r = bias_grad(c, c->d2, H, H, c->g + c->off_b[1])) return r;
+    hipLaunchKernelGGL((conv_fwd_kernel<32, 2, kMT, 1>), dim3(grid), dim3(256), 0, c->stream, c->d2, c->pkt[0], nullptr,
                        (void*)c->d1, H, c->nx, c->ny, c->nz);
     hipLaunchKernelGGL(relu_back_kernel, dim3(blocks_for(V * H)), dim3(256), 0, c->stream, c->d1, c->a1, V * H);
     // layer 1
     if (int r = wgrad(c, x, kInC, c->d1, H, H, c->g + c->off_w[0])) return r;
-    hipLaunchKernelGGL(bias_grad_kernel, dim3(H), dim3(256), 0, c->stream, c->d1, H, H, V, c->g + c->off_b[0]);
+    if (int r = bias_grad(c, c->d1, H, H, c->g + c->off_b[0])) return r;
     CHK(hipGetLastError());
     if (loss_out) {
         std::vector<float> lp(c->loss_blocks);
@@ -478,6 +548,9 @@ int lbm_closure_create(int32_t nx, int32_t ny, int32_t nz, int32_t hidden, uint6
     c->V = (long)nx * ny * nz;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
         return bail(fail(LBM_ERR_HIP, "hipStreamCreate failed"));
+    if (hipEventCreateWithFlags(&c->ev_before, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_after, hipEventDisableTiming) != hipSuccess)
+        return bail(fail(LBM_ERR_HIP, "hipEventCreate failed"));
     const int H = hidden;
     c->cin[0] = kInC, c->cout[0] = H;
     c->cin[1] = H, c->cout[1] = H;
@@ -544,6 +617,8 @@ int lbm_closure_destroy(lbm_closure_handle c) {
     for (void* p : c->allocs) (void)hipFree(p);
     for (auto* p : c->sx) (void)hipFree(p);
     for (auto* p : c->sy) (void)hipFree(p);
+    if (c->ev_before) (void)hipEventDestroy(c->ev_before);
+    if (c->ev_after) (void)hipEventDestroy(c->ev_after);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return LBM_OK;
@@ -659,7 +734,7 @@ int lbm_closure_loss_and_grad(lbm_closure_handle c, const double* x, const doubl
 int lbm_closure_predict(lbm_closure_handle c, const double* x, double* y) {
     if (!c || !x || !y) return fail(LBM_ERR_INVALID, "null argument");
     if (int r = upload_x(c, x, c->x)) return r;
-    if (int r = forward(c, c->x)) return r;
+    if (int r = forward(c, c->x, c->stream)) return r;
     std::vector<float> h(c->V * kOutC);
     CHK(hipMemcpyAsync(h.data(), c->pred, h.size() * sizeof(float), hipMemcpyDeviceToHost, c->stream));
     CHK(hipStreamSynchronize(c->stream));
@@ -673,14 +748,18 @@ int lbm_closure_apply(lbm_closure_handle c, lbm_handle sim, double alpha) {
     if (sim->fp64) return fail(LBM_ERR_UNSUPPORTED, "the closure corrects FP32 simulations");
     if (sim->cfg.nx != c->nx || sim->cfg.ny != c->ny || sim->cfg.nz != c->nz)
         return fail(LBM_ERR_INVALID, "closure and simulation grids differ");
-    CHK(hipStreamSynchronize(sim->stream));
-    hipLaunchKernelGGL(gather_input_kernel, dim3(blocks_for(c->V)), dim3(256), 0, c->stream, sim->d32, c->V,
+    // Runs on the simulation's stream, after the closure's pending work and
+    // before any later closure work; the host does not wait.
+    CHK(hipEventRecord(c->ev_before, c->stream));
+    CHK(hipStreamWaitEvent(sim->stream, c->ev_before, 0));
+    hipLaunchKernelGGL(gather_input_kernel, dim3(blocks_for(c->V)), dim3(256), 0, sim->stream, sim->d32, c->V,
                        c->in_scale, c->x);
-    if (int r = forward(c, c->x)) return r;
-    hipLaunchKernelGGL(apply_kernel, dim3(blocks_for(c->V)), dim3(256), 0, c->stream, sim->d32, c->V, c->pred,
+    if (int r = forward(c, c->x, sim->stream)) return r;
+    hipLaunchKernelGGL(apply_kernel, dim3(blocks_for(c->V)), dim3(256), 0, sim->stream, sim->d32, c->V, c->pred,
                        c->out_scale, (float)alpha);
     CHK(hipGetLastError());
-    CHK(hipStreamSynchronize(c->stream));
+    CHK(hipEventRecord(c->ev_after, sim->stream));
+    CHK(hipStreamWaitEvent(c->stream, c->ev_after, 0));
     return LBM_OK;
 }
 
@@ -691,7 +770,7 @@ int lbm_closure_time_forward(lbm_closure_handle c, int32_t n, double* ms) {
     CHK(hipEventCreate(&b));
     CHK(hipEventRecord(a, c->stream));
     for (int i = 0; i < n; ++i)
-        if (int r = forward(c, c->x)) return r;
+        if (int r = forward(c, c->x, c->stream)) return r;
     CHK(hipEventRecord(b, c->stream));
     CHK(hipEventSynchronize(b));
     float f = 0.f;
