@@ -175,6 +175,16 @@ int lbm_diagnostics(lbm_handle h, lbm_diag* out);
  * [V][3] interleaved. Computed on the device in FP64. */
 int lbm_order_director(lbm_handle h, double* order, double* director);
 
+/* Plaquettes pierced by a disclination line (DefectFinder::ComputeWindingNumbers,
+ * src/analysis/defect_finder.tpp:12-162): 1 where the director of
+ * lbm_order_director, carried around the plaquette with a sign flip whenever
+ * n1 . n2 <= 0 (FlipN, src/analysis/defect_fields.cc:15-24), returns flipped,
+ * else 0. Layouts as DefectFields (src/analysis/defect_fields.h:17-30):
+ * def_x: nx*(ny-1)*(nz-1) plaquettes normal to x, index (z*(ny-1) + y)*nx + x;
+ * def_y: (nx-1)*ny*(nz-1), index (z*ny + y)*(nx-1) + x;
+ * def_z: (nx-1)*(ny-1)*nz, index (z*(ny-1) + y)*(nx-1) + x. */
+int lbm_defect_faces(lbm_handle h, uint8_t* def_x, uint8_t* def_y, uint8_t* def_z);
+
 /* ---- timing and kernel selection ------------------------------------------ */
 /* Run n full steps and return the elapsed device time (ms) measured with HIP
  * events on the handle's stream. */
@@ -244,7 +254,8 @@ int lbm_batch_autotune(lbm_batch_handle b, int32_t n, double* ms_out, int32_t ma
  * kernels with FP32 accumulation; FP32 master weights and Adam state.
  * Fields on the host are channels-first FP64: x is 8 blocks of V (Q xx, xy,
  * xz, yy, yz, then ux, uy, uz), y is 5 blocks of V. nx must be a multiple of
- * 16; hidden must be 32. Parameters, layer by layer: W[27][cin][cout] with
+ * 16 and ny, nz multiples of 4 (the convolutions work on 16 x 4 x 4 cell
+ * tiles); hidden must be 32. Parameters, layer by layer: W[27][cin][cout] with
  * stencil offset (dz+1)*9 + (dy+1)*3 + (dx+1), then b[cout]. */
 typedef struct lbm_closure* lbm_closure_handle;
 int lbm_closure_create(int32_t nx, int32_t ny, int32_t nz, int32_t hidden, uint64_t seed,

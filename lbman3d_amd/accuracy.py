@@ -10,8 +10,9 @@ the yardstick. Three runs start from the same seeded state:
 Every `every` steps the relative L2 error of Q, u and rho - rho0 of fp32 and
 fp64-ulp against fp64 is printed as one JSON line.
 
-usage: python -m lbman3d_amd.accuracy --workload channel   (20x100x100, 10^4 steps)
-       python -m lbman3d_amd.accuracy --workload bandwidth (256^3, 2000 steps)
+usage: python -m lbman3d_amd.accuracy --workload channel          (20x100x100, 10^4 steps)
+       python -m lbman3d_amd.accuracy --workload channel-passive  (the same with zeta = 0)
+       python -m lbman3d_amd.accuracy --workload bandwidth        (256^3, 2000 steps)
 """
 from __future__ import annotations
 
@@ -28,6 +29,10 @@ WORKLOADS = {
     # ChannelConfig 20x100x100, params.h defaults with lambda 0.5, centred.
     "channel": dict(dims=(20, 100, 100), bc=abi.ChannelConfig, params=abi.Params(LAMBDA=0.5),
                     adv=abi.ADV_CENTRED, steps=10000, every=1000, seed=1),
+    # The same channel without activity (zeta 0): relaxational, not chaotic,
+    # so the FP32 error is the arithmetic's own and is not amplified.
+    "channel-passive": dict(dims=(20, 100, 100), bc=abi.ChannelConfig, params=abi.Params(LAMBDA=0.5, ALPHA=0.0),
+                            adv=abi.ADV_CENTRED, steps=10000, every=1000, seed=1),
     # FullyPeriodic 256^3, zeta 2x default, lambda 0.7, upwind.
     "bandwidth": dict(dims=(256, 256, 256), bc=abi.FullyPeriodicConfig,
                       params=abi.Params(ALPHA=0.046, LAMBDA=0.7), adv=abi.ADV_UPWIND, steps=2000, every=250,

@@ -24,6 +24,7 @@ SOURCES = {
     "kernels_fp32.hip": ["-ffp-contract=fast", "-munsafe-fp-atomics"],
     "batch.hip": ["-ffp-contract=fast", "-x", "hip"],
     "closure.hip": ["-ffp-contract=fast"],
+    "defects.hip": ["-ffp-contract=off"],
 }
 EXTRA: list[str] = []
 

@@ -46,57 +46,56 @@ __device__ __forceinline__ float from_bf16(__bf16 v) { return (float)v; }
 
 __device__ __forceinline__ int wrap(int t, int n) { return t < 0 ? t + n : (t >= n ? t - n : t); }
 
-// Neighbour cell of (x, y, z) for stencil offset `off` on a periodic grid.
-__device__ __forceinline__ int nbr(int x, int y, int z, int off, int nx, int ny, int nz) {
-    const int dz = off / 9 - 1, dy = (off / 3) % 3 - 1, dx = off % 3 - 1;
-    return (wrap(z + dz, nz) * ny + wrap(y + dy, ny)) * nx + wrap(x + dx, nx);
-}
-
 // ---- forward convolution (also the input-gradient convolution) -------------
-// One wavefront: MT row tiles of 16 consecutive cells x (16 * NT) output
-// channels; each weight fragment is loaded once per K step and used for all
-// MT row tiles. Cin is 8 (four stencil offsets per 32-wide K step) or 32 (one
-// offset per K step).
+// A block owns a 16 x 4 x 4 tile of output cells. It stages the periodic
+// 18 x 6 x 6 halo of the input (all CIN channels, BF16) in LDS once, then
+// every stencil offset's A fragment is an LDS read; the 27-fold reuse of
+// each input cell stays on chip. Wavefront w computes output plane z = w of
+// the tile: 4 row tiles (y) of 16 cells (x) x (16 * NT) output channels,
+// each weight fragment loaded once per K step for all 4 row tiles. Cin is 8
+// (four stencil offsets per 32-wide K step) or 32 (one offset per K step).
 // EPI: 0 = store BF16 with bias + ReLU, 1 = store BF16 linear (no bias),
 // 2 = store FP32 channels-first [co][cell] with bias (final layer).
-template <int CIN, int NT, int MT, int EPI>
+constexpr int kTX = 16, kTY = 4, kTZ = 4;
+constexpr int kHX = kTX + 2, kHY = kTY + 2, kHZ = kTZ + 2, kNH = kHX * kHY * kHZ;
+
+template <int CIN, int NT, int EPI>
 __global__ void __launch_bounds__(256)
 conv_fwd_kernel(const __bf16* __restrict__ in, const bf16x8* __restrict__ wpk, const float* __restrict__ bias,
                 void* __restrict__ out, int cout, int nx, int ny, int nz) {
-    const int lane = threadIdx.x & 63;
-    const long wave = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const long V = (long)nx * ny * nz;
-    const long c0 = wave * 16 * MT;
-    if (c0 >= V) return;
-    const int m = lane & 15, kb = lane >> 4;
-    int xs[MT], ys[MT], zs[MT];
-    bool ok[MT];
-#pragma unroll
-    for (int t = 0; t < MT; ++t) {
-        const long cm = c0 + 16 * t + m;
-        ok[t] = cm < V;
-        xs[t] = (int)(cm % nx);
-        ys[t] = (int)((cm / nx) % ny);
-        zs[t] = (int)(cm / ((long)nx * ny));
+    constexpr int CV = CIN / 8;          // 16-byte vectors per cell
+    constexpr int STRIDE = CV + 1;       // padded cell stride in LDS (16-byte units)
+    __shared__ bf16x8 tile[kNH * STRIDE];
+    const int ntx = nx / kTX, nty = ny / kTY;
+    const int bx = blockIdx.x % ntx, by = (blockIdx.x / ntx) % nty, bz = blockIdx.x / (ntx * nty);
+    const int x0 = bx * kTX - 1, y0 = by * kTY - 1, z0 = bz * kTZ - 1;
+    for (int v = threadIdx.x; v < kNH * CV; v += 256) {
+        const int hc = v / CV, q = v % CV;
+        const int hx = hc % kHX, hy = (hc / kHX) % kHY, hz = hc / (kHX * kHY);
+        const long g = ((long)wrap(z0 + hz, nz) * ny + wrap(y0 + hy, ny)) * nx + wrap(x0 + hx, nx);
+        tile[hc * STRIDE + q] = *reinterpret_cast<const bf16x8*>(in + g * CIN + q * 8);
     }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int m = lane & 15, kb = lane >> 4;
     constexpr int KCH = CIN == 8 ? 7 : 27;   // K steps of 32 (8-channel input: 28 offsets, last zero)
-    f32x4 acc[MT][NT];
+    f32x4 acc[kTY][NT];
 #pragma unroll
-    for (int t = 0; t < MT; ++t)
+    for (int t = 0; t < kTY; ++t)
 #pragma unroll
         for (int n = 0; n < NT; ++n) acc[t][n] = (f32x4){0.f, 0.f, 0.f, 0.f};
     for (int kc = 0; kc < KCH; ++kc) {
         const int off = CIN == 8 ? kc * 4 + kb : kc;
-        const int ch = CIN == 8 ? 0 : kb * 8;
+        const int q = CIN == 8 ? 0 : kb;
+        const int dz = off / 9, dy = (off / 3) % 3, dx = off % 3;   // halo coordinates shift by +1
         bf16x8 b[NT];
 #pragma unroll
         for (int n = 0; n < NT; ++n) b[n] = wpk[((long)kc * NT + n) * 64 + lane];
 #pragma unroll
-        for (int t = 0; t < MT; ++t) {
+        for (int t = 0; t < kTY; ++t) {
             bf16x8 a;
-            if (off < kOffsets && ok[t]) {
-                const int nc = nbr(xs[t], ys[t], zs[t], off, nx, ny, nz);
-                a = *reinterpret_cast<const bf16x8*>(in + (long)nc * CIN + ch);
+            if (off < kOffsets) {
+                a = tile[(((w + dz) * kHY + (t + dy)) * kHX + (m + dx)) * STRIDE + q];
             } else {
 #pragma unroll
                 for (int j = 0; j < 8; ++j) a[j] = to_bf16(0.f);
@@ -106,18 +105,19 @@ conv_fwd_kernel(const __bf16* __restrict__ in, const bf16x8* __restrict__ wpk, c
                 acc[t][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[n], acc[t][n], 0, 0, 0);
         }
     }
-    // D fragment: lane holds rows 4*kb + r (cells), column n = m (channel).
+    // D fragment: lane holds rows 4*kb + r (x within the row tile), column n*16 + m (channel).
+    const long V = (long)nx * ny * nz;
 #pragma unroll
     for (int n = 0; n < NT; ++n) {
         const int co = n * 16 + m;
         if (co >= cout) continue;
         const float bco = (EPI == 1) ? 0.f : bias[co];
 #pragma unroll
-        for (int t = 0; t < MT; ++t) {
+        for (int t = 0; t < kTY; ++t) {
+            const long row = ((long)(bz * kTZ + w) * ny + by * kTY + t) * nx + bx * kTX;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const long cell = c0 + 16 * t + 4 * kb + r;
-                if (cell >= V) continue;
+                const long cell = row + 4 * kb + r;
                 float v = acc[t][n][r] + bco;
                 if (EPI == 0) v = v > 0.f ? v : 0.f;
                 if (EPI == 2) reinterpret_cast<float*>(out)[(long)co * V + cell] = v;
@@ -421,18 +421,18 @@ int repack(lbm_closure* c) {
     return LBM_OK;
 }
 
-constexpr int kMT = 4;   // row tiles of 16 cells per wavefront in the convolutions
-unsigned conv_grid(long V) { return blocks_for((V + 16 * kMT - 1) / (16 * kMT) * 64); }
+// One block per 16 x 4 x 4 output tile (the grid is a whole number of tiles).
+unsigned conv_grid(long V) { return (unsigned)(V / (kTX * kTY * kTZ)); }
 
 // Forward pass of the input x (BF16 [cell][8]) into c->pred, on stream s.
 int forward(lbm_closure* c, const __bf16* x, hipStream_t s) {
     const unsigned grid = conv_grid(c->V);
     const int H = c->hidden;
-    hipLaunchKernelGGL((conv_fwd_kernel<8, 2, kMT, 0>), dim3(grid), dim3(256), 0, s, x, c->pk[0],
+    hipLaunchKernelGGL((conv_fwd_kernel<8, 2, 0>), dim3(grid), dim3(256), 0, s, x, c->pk[0],
                        c->w + c->off_b[0], (void*)c->a1, H, c->nx, c->ny, c->nz);
-    hipLaunchKernelGGL((conv_fwd_kernel<32, 2, kMT, 0>), dim3(grid), dim3(256), 0, s, c->a1, c->pk[1],
+    hipLaunchKernelGGL((conv_fwd_kernel<32, 2, 0>), dim3(grid), dim3(256), 0, s, c->a1, c->pk[1],
                        c->w + c->off_b[1], (void*)c->a2, H, c->nx, c->ny, c->nz);
-    hipLaunchKernelGGL((conv_fwd_kernel<32, 1, kMT, 2>), dim3(grid), dim3(256), 0, s, c->a2, c->pk[2],
+    hipLaunchKernelGGL((conv_fwd_kernel<32, 1, 2>), dim3(grid), dim3(256), 0, s, c->a2, c->pk[2],
                        c->w + c->off_b[2], (void*)c->pred, kOutC, c->nx, c->ny, c->nz);
     CHK(hipGetLastError());
     return LBM_OK;
@@ -485,13 +485,13 @@ int loss_and_grad(lbm_closure* c, const __bf16* x, const float* y, double* loss_
     if (int r = wgrad(c, c->a2, H, c->d3, 8, kOutC, c->g + c->off_w[2])) return r;
     if (int r = bias_grad(c, c->d3, 8, kOutC, c->g + c->off_b[2])) return r;
     const unsigned grid = conv_grid(V);
-    hipLaunchKernelGGL((conv_fwd_kernel<8, 2, kMT, 1>), dim3(grid), dim3(256), 0, c->stream, c->d3, c->pkt[1], nullptr,
+    hipLaunchKernelGGL((conv_fwd_kernel<8, 2, 1>), dim3(grid), dim3(256), 0, c->stream, c->d3, c->pkt[1], nullptr,
                        (void*)c->d2, H, c->nx, c->ny, c->nz);
     hipLaunchKernelGGL(relu_back_kernel, dim3(blocks_for(V * H)), dim3(256), 0, c->stream, c->d2, c->a2, V * H);
     // layer 2
     if (int r = wgrad(c, c->a1, H, c->d2, H, H, c->g + c->off_w[1])) return r;
     if (int r = bias_grad(c, c->d2, H, H, c->g + c->off_b[1])) return r;
-    hipLaunchKernelGGL((conv_fwd_kernel<32, 2, kMT, 1>), dim3(grid), dim3(256), 0, c->stream, c->d2, c->pkt[0], nullptr,
+    hipLaunchKernelGGL((conv_fwd_kernel<32, 2, 1>), dim3(grid), dim3(256), 0, c->stream, c->d2, c->pkt[0], nullptr,
                        (void*)c->d1, H, c->nx, c->ny, c->nz);
     hipLaunchKernelGGL(relu_back_kernel, dim3(blocks_for(V * H)), dim3(256), 0, c->stream, c->d1, c->a1, V * H);
     // layer 1
@@ -534,8 +534,8 @@ int lbm_closure_create(int32_t nx, int32_t ny, int32_t nz, int32_t hidden, uint6
     if (!out) return fail(LBM_ERR_INVALID, "null argument");
     *out = nullptr;
     if (hidden != 32) return fail(LBM_ERR_UNSUPPORTED, "the closure kernels are built for 32 hidden channels");
-    if (nx < 1 || ny < 1 || nz < 1 || (long)nx * ny * nz % 16 != 0 || nx % 16 != 0)
-        return fail(LBM_ERR_INVALID, "closure grid must have nx a multiple of 16");
+    if (nx < 16 || ny < 4 || nz < 4 || nx % 16 != 0 || ny % 4 != 0 || nz % 4 != 0)
+        return fail(LBM_ERR_INVALID, "closure grid must have nx a multiple of 16 and ny, nz multiples of 4");
     lbm_closure* c = new lbm_closure();
     auto bail = [&](int r) {
         lbm_closure_destroy(c);
