@@ -20,7 +20,10 @@
 // the committed fixtures tests/golden/*.npz and a live run of oracle/_ref
 // where the reference is present (tests/test_oracle_golden.py), and against
 // the closed forms the reference's integration tests use (Poiseuille profile,
-// equilibrium scalar order, conservation; tests/test_invariants.py).
+// equilibrium scalar order, conservation; tests/test_invariants.py), and its
+// pointwise entry points against the closed forms of the reference's unit
+// tests (lattice tables, equilibrium and Guo moments, BGK fixed point,
+// polynomial stencils; tests/test_oracle_kat.py).
 
 #include <cmath>
 #include <cstdint>
