@@ -144,10 +144,14 @@ def test_device_faces_match_restatement():
 
 @pytest.mark.gpu
 def test_device_finds_a_seeded_wedge_line(tmp_path):
+    """A +1/2 wedge line along x uploaded as Q = S (n n - I/3). The line is
+    placed so that no cell has n_z = 0: the reference's eigenvector formula
+    (QtensorToOrderDirector) degenerates for directors in the xy plane, and
+    the director is taken from Q on the device exactly as it does."""
     from lbman3d_amd.solver import DeviceSolver
     print("case defect wedge", flush=True)
-    nx, ny, nz = 16, 16, 8
-    d = wedge(nx, ny, nz, 0.5, "z", (7.5, 7.5))
+    nx, ny, nz = 8, 16, 8
+    d = wedge(nx, ny, nz, 0.5, "x", (7.5, 3.5))          # centre (y, z)
     S = 0.5
     q = np.stack([S * (d[:, 0] ** 2 - 1 / 3), S * d[:, 0] * d[:, 1], S * d[:, 0] * d[:, 2],
                   S * (d[:, 1] ** 2 - 1 / 3), S * d[:, 1] * d[:, 2]])
@@ -156,5 +160,5 @@ def test_device_finds_a_seeded_wedge_line(tmp_path):
         lines = defects.find_disclinations(s)
         path = defects.export_disclinations(s, str(tmp_path), 0, 1)
     assert len(lines) == 1
-    np.testing.assert_array_equal(lines[0], [[7.5, 7.5, float(z)] for z in range(nz)])
+    np.testing.assert_array_equal(lines[0], [[float(x), 7.5, 3.5] for x in range(nx)])
     assert os.path.basename(path) == "disclinations_0.vtkhdf"
