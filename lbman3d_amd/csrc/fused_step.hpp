@@ -820,8 +820,9 @@ constexpr TileShape kTiles[] = {
     // is a smaller share of the interior (64x8: 148 of 512, 32x16: 100 of 512)
     {1, 64, 8, 64, "pf:64x8x64"},     {1, 32, 16, 64, "pf:32x16x64"},   {1, 64, 8, 32, "pf:64x8x32"},
     {1, 32, 16, 32, "pf:32x16x32"},
-    // longer z chunks: fewer halo planes per chunk, one round of blocks at 256^3
-    {2, 64, 4, 128, "pf-nt:64x4x128"},
+    // longer z chunks: fewer halo planes per chunk, and at 256^3 every block
+    // is resident at once (512 blocks of 4 waves, or 256 of 8 waves)
+    {2, 64, 4, 128, "pf-nt:64x4x128"}, {2, 32, 8, 128, "pf-nt:32x8x128"}, {2, 64, 8, 128, "pf-nt:64x8x128"},
 };
 constexpr int kNumTiles = sizeof(kTiles) / sizeof(kTiles[0]);
 
@@ -874,7 +875,9 @@ void launch_fused_step_f32(const Fields<float>& f, const BCDesc<float>& bc, cons
         case 10: launch_tile<1, 32, 16, 64>(f, bc, k, pull, aux, s); break;
         case 11: launch_tile<1, 64, 8, 32>(f, bc, k, pull, aux, s); break;
         case 12: launch_tile<1, 32, 16, 32>(f, bc, k, pull, aux, s); break;
-        default: launch_tile<2, 64, 4, 128>(f, bc, k, pull, aux, s); break;
+        case 13: launch_tile<2, 64, 4, 128>(f, bc, k, pull, aux, s); break;
+        case 14: launch_tile<2, 32, 8, 128>(f, bc, k, pull, aux, s); break;
+        default: launch_tile<2, 64, 8, 128>(f, bc, k, pull, aux, s); break;
     }
 }
 
