@@ -41,7 +41,7 @@ struct BatchTile {
     const char* name;
 };
 constexpr BatchTile kBatchTiles[] = {{32, 8, 32, "32x8x32"}, {16, 16, 32, "16x16x32"}, {32, 8, 16, "32x8x16"},
-                                     {16, 16, 16, "16x16x16"}};
+                                     {16, 16, 16, "16x16x16"}, {16, 16, 8, "16x16x8"},   {16, 8, 16, "16x8x16"}};
 constexpr int kNumBatchTiles = sizeof(kBatchTiles) / sizeof(kBatchTiles[0]);
 
 template <int TX, int TY, int ZC>
@@ -131,7 +131,9 @@ int grid_for_tile(int tile, int total) {
         case 0: occ = occupancy_blocks<32, 8, 32>(); break;
         case 1: occ = occupancy_blocks<16, 16, 32>(); break;
         case 2: occ = occupancy_blocks<32, 8, 16>(); break;
-        default: occ = occupancy_blocks<16, 16, 16>(); break;
+        case 3: occ = occupancy_blocks<16, 16, 16>(); break;
+        case 4: occ = occupancy_blocks<16, 16, 8>(); break;
+        default: occ = occupancy_blocks<16, 8, 16>(); break;
     }
     return std::max(1, std::min(total, cus * occ));
 }
@@ -191,7 +193,9 @@ void launch_step(lbm_batch* b, int aux) {
         case 0: launch_batch<32, 8, 32>(b->d_desc, n, b->total, b->parity, aux, b->grid, b->stream); break;
         case 1: launch_batch<16, 16, 32>(b->d_desc, n, b->total, b->parity, aux, b->grid, b->stream); break;
         case 2: launch_batch<32, 8, 16>(b->d_desc, n, b->total, b->parity, aux, b->grid, b->stream); break;
-        default: launch_batch<16, 16, 16>(b->d_desc, n, b->total, b->parity, aux, b->grid, b->stream); break;
+        case 3: launch_batch<16, 16, 16>(b->d_desc, n, b->total, b->parity, aux, b->grid, b->stream); break;
+        case 4: launch_batch<16, 16, 8>(b->d_desc, n, b->total, b->parity, aux, b->grid, b->stream); break;
+        default: launch_batch<16, 8, 16>(b->d_desc, n, b->total, b->parity, aux, b->grid, b->stream); break;
     }
 }
 
