@@ -823,6 +823,9 @@ constexpr TileShape kTiles[] = {
     // longer z chunks: fewer halo planes per chunk, and at 256^3 every block
     // is resident at once (512 blocks of 4 waves, or 256 of 8 waves)
     {2, 64, 4, 128, "pf-nt:64x4x128"}, {2, 32, 8, 128, "pf-nt:32x8x128"}, {2, 64, 8, 128, "pf-nt:64x8x128"},
+    // short z chunks: enough blocks to fill the CUs on thin or small grids
+    // (a 256 x 256 x 32 slab, 128^3)
+    {2, 64, 4, 16, "pf-nt:64x4x16"},   {2, 32, 8, 16, "pf-nt:32x8x16"},
 };
 constexpr int kNumTiles = sizeof(kTiles) / sizeof(kTiles[0]);
 
@@ -877,7 +880,9 @@ void launch_fused_step_f32(const Fields<float>& f, const BCDesc<float>& bc, cons
         case 12: launch_tile<1, 32, 16, 32>(f, bc, k, pull, aux, s); break;
         case 13: launch_tile<2, 64, 4, 128>(f, bc, k, pull, aux, s); break;
         case 14: launch_tile<2, 32, 8, 128>(f, bc, k, pull, aux, s); break;
-        default: launch_tile<2, 64, 8, 128>(f, bc, k, pull, aux, s); break;
+        case 15: launch_tile<2, 64, 8, 128>(f, bc, k, pull, aux, s); break;
+        case 16: launch_tile<2, 64, 4, 16>(f, bc, k, pull, aux, s); break;
+        default: launch_tile<2, 32, 8, 16>(f, bc, k, pull, aux, s); break;
     }
 }
 
