@@ -323,15 +323,54 @@ __device__ __forceinline__ float stage_value(const StageA& s, int m) {
     return m < 5 ? s.qn[m] : (m < 10 ? s.sg[m - 5] : s.ta[m - 10]);
 }
 
+// Cell classes on domains with walls. A face is open unless both its Q and
+// its velocity tags are periodic. A cell at least two cells away from every
+// open face has its whole dependency cone (inputs two cells out, pull
+// sources and force stencil one cell out) inside the domain and away from
+// every ghost, clamp or wall slot, so the periodic specialisation computes it
+// exactly: class 0. The others lie in the shell of an axis, x first: class
+// 1 + axis.
+//
+// Write modes of a launch: 0 every cell, 1 class 0 only, 2 + a the shell of
+// axis a. The class is formed once per column and once per plane with
+// integer arithmetic, with no branches: boolean tests of the face tags
+// evaluated inside a lane-divergent short circuit were seen to be reused by
+// the compiler, as lane masks of that branch, in the ghost tests of the
+// force stage (q_pair), which then took a periodic face for a wall.
+struct CellMask {
+    int mode, col, zlo, zhi, nz;
+    __device__ __forceinline__ bool writes(int z) const {
+        const int zin = (zlo & (z < 2)) | (zhi & (z >= nz - 2));
+        const int cls = col + (col == 0) * 3 * zin;
+        return (mode == 0) | ((mode == 1) & (cls == 0)) | ((mode >= 2) & (cls == mode - 1));
+    }
+};
+__device__ __forceinline__ CellMask cell_mask(const BCDesc<float>& bc, int mode, int x, int y) {
+    int open[6];
+#pragma unroll
+    for (int face = 0; face < 6; ++face)
+        open[face] = (bc.qkind[face] != QB_PERIODIC) | (bc.ukind[face] != UB_PERIODIC);
+    const int inx = (open[0] & (x < 2)) | (open[1] & (x >= bc.n[0] - 2));
+    const int iny = (open[2] & (y < 2)) | (open[3] & (y >= bc.n[1] - 2));
+    CellMask m;
+    m.mode = mode;
+    m.col = inx + (1 - inx) * 2 * iny;
+    m.zlo = open[4];
+    m.zhi = open[5];
+    m.nz = bc.n[2];
+    return m;
+}
+
 // First pipeline, one tile: z planes [tz*ZC, tz*ZC + ZC) of the TX x TY
-// column tile (tx, ty). LDS buffers come from the calling kernel so that a
-// kernel instantiating several variants does not allocate LDS per variant.
+// column tile (tx, ty), writing the cells that `mode` selects. LDS buffers
+// come from the calling kernel so that a kernel instantiating several
+// variants does not allocate LDS per variant.
 template <int TX, int TY, int ZC, bool PERIODIC>
 __device__ __forceinline__ void fused_body(const Fields<float>& f, const BCDesc<float>& bc, const Phys<float>& k,
                                            int pull, int aux, int tx, int ty, int tz,
                                            float (*sIn)[TileGeom<TX, TY>::NI],
                                            float (*sA)[kNumA][TileGeom<TX, TY>::NA],
-                                           float (*sRing)[kNumIn][TileGeom<TX, TY>::NR]) {
+                                           float (*sRing)[kNumIn][TileGeom<TX, TY>::NR], int mode = 0) {
     using G = TileGeom<TX, TY>;
     const int nx = bc.n[0], ny = bc.n[1], nz = bc.n[2];
     const int x0 = tx * TX, y0 = ty * TY;
@@ -355,6 +394,7 @@ __device__ __forceinline__ void fused_body(const Fields<float>& f, const BCDesc<
     const int acell = ib * G::AX + ia;
     const int gx = x0 + ia - 1, gy = y0 + ib - 1;
     const bool active = gx < nx && gy < ny;
+    const CellMask wmask = cell_mask(bc, mode, gx, gy);
     // Stage-A ring column (threads t < NR) and input outer-ring column (last NO threads).
     const bool has_ring = t < G::NR;
     int ra = 0, rb = 0;
@@ -543,17 +583,19 @@ __device__ __forceinline__ void fused_body(const Fields<float>& f, const BCDesc<
             float rho, u[3];
             collide_shifted(k, pop, F, rho, u);
             const unsigned vs4 = (unsigned)f.vs * 4u, c4 = c * 4u;
+            if (wmask.writes(pb)) {
 #pragma unroll
-            for (int j = 0; j < kQ; ++j) sto(f.gn, j * vs4 + c4, pop[j]);
+                for (int j = 0; j < kQ; ++j) sto(f.gn, j * vs4 + c4, pop[j]);
 #pragma unroll
-            for (int m = 0; m < 3; ++m) sto(f.un[m], c4, u[m]);
+                for (int m = 0; m < 3; ++m) sto(f.un[m], c4, u[m]);
 #pragma unroll
-            for (int m = 0; m < 5; ++m) sto(f.qn[m], c4, qc[m]);
-            if (aux) {
-                sto(f.rho, c4, rho);
-                sto(f.F[0], c4, F[0]);
-                sto(f.F[1], c4, F[1]);
-                sto(f.F[2], c4, F[2]);
+                for (int m = 0; m < 5; ++m) sto(f.qn[m], c4, qc[m]);
+                if (aux) {
+                    sto(f.rho, c4, rho);
+                    sto(f.F[0], c4, F[0]);
+                    sto(f.F[1], c4, F[1]);
+                    sto(f.F[2], c4, F[2]);
+                }
             }
         }
         // Plane pb becomes the lower z neighbour of the next plane.
@@ -566,7 +608,7 @@ __device__ __forceinline__ void fused_body(const Fields<float>& f, const BCDesc<
 template <int TX, int TY, int ZC, bool PERIODIC>
 __global__ void __launch_bounds__(TX * TY)
 fused_step_kernel(Fields<float> f, BCDesc<float> bc, Phys<float> k, int pull, int aux, int ntx, int nty,
-                  int nzc) {
+                  int nzc, int mode) {
     using G = TileGeom<TX, TY>;
     __shared__ float sIn[kNumIn][G::NI];          // input plane pa, extended region
     __shared__ float sA[2][kNumA][G::NA];         // stage-A results, planes pa and pa - 1
@@ -579,19 +621,46 @@ fused_step_kernel(Fields<float> f, BCDesc<float> bc, Phys<float> k, int pull, in
     const int L = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
     if (L >= nblk) return;
     const int tx = L % ntx, ty = (L / ntx) % nty, tz = L / (ntx * nty);
-    fused_body<TX, TY, ZC, PERIODIC>(f, bc, k, pull, aux, tx, ty, tz, sIn, sA, sRing);
+    fused_body<TX, TY, ZC, PERIODIC>(f, bc, k, pull, aux, tx, ty, tz, sIn, sA, sRing, mode);
+}
+
+// Tiles of one wall shell: along `axis`, the tile indices idx[0..cnt) that
+// hold shell cells; along the other two axes, every tile.
+struct ShellTiles {
+    int axis, cnt, idx[4], nt[3];
+};
+
+// The shell cells of one axis (write mode 2 + axis) with the general kernel.
+template <int TX, int TY, int ZC>
+__global__ void __launch_bounds__(TX * TY)
+fused_shell_kernel(Fields<float> f, BCDesc<float> bc, Phys<float> k, int pull, int aux, ShellTiles st) {
+    using G = TileGeom<TX, TY>;
+    __shared__ float sIn[kNumIn][G::NI];
+    __shared__ float sA[2][kNumA][G::NA];
+    __shared__ float sRing[3][kNumIn][G::NR];
+    const int L = blockIdx.x;
+    const int a1 = (st.axis + 1) % 3, a2 = (st.axis + 2) % 3;
+    if (L >= st.cnt * st.nt[a1] * st.nt[a2]) return;
+    int t[3];
+    t[st.axis] = st.idx[L % st.cnt];
+    t[a1] = (L / st.cnt) % st.nt[a1];
+    t[a2] = L / (st.cnt * st.nt[a1]);
+    fused_body<TX, TY, ZC, false>(f, bc, k, pull, aux, t[0], t[1], t[2], sIn, sA, sRing, 2 + st.axis);
 }
 
 // Second pipeline: the same computation with one stage-A plane in LDS (the
 // force stage reads plane pb while the fresh plane pa waits in registers and
 // is published after a barrier), and with every global load issued one z
 // plane ahead of its use: the pull gather of plane pb and the input planes
-// pa + 2 are in flight while the Q update of plane pa computes.
-template <int TX, int TY, int ZC, bool PERIODIC, int WPE, bool UPWIND, bool NT>
+// pa + 2 are in flight while the Q update of plane pa computes. Only the
+// periodic specialisation is built; on a domain with walls (MASKED) it
+// writes the class-0 cells (write mode 1) and the shell launches do the rest.
+template <int TX, int TY, int ZC, bool MASKED, int WPE, bool UPWIND, bool NT>
 __global__ void __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(WPE, WPE)))
 fused_step_kernel_pf(Fields<float> f, BCDesc<float> bc, Phys<float> k, int pull, int aux, int ntx, int nty,
-                     int nzc) {
+                     int nzc, int mode) {
     using G = TileGeom<TX, TY>;
+    constexpr bool PERIODIC = true;
     __shared__ float sIn[kNumIn][G::NI];           // input plane pa, extended region
     __shared__ float sA[kNumA][G::NA];             // stage-A results of plane pa - 1
     __shared__ float sRing[4][kNumIn][G::NR];      // ring-column inputs, planes mod 4
@@ -620,6 +689,7 @@ fused_step_kernel_pf(Fields<float> f, BCDesc<float> bc, Phys<float> k, int pull,
     const int acell = ib * G::AX + ia;
     const int gx = x0 + ia - 1, gy = y0 + ib - 1;
     const bool active = gx < nx && gy < ny;
+    const CellMask wmask = cell_mask(bc, mode, gx, gy);
     const bool has_ring = t < G::NR;
     int ra = 0, rb = 0;
     if (has_ring) ring_col(t, G::AX, G::AY, ra, rb);
@@ -762,17 +832,19 @@ fused_step_kernel_pf(Fields<float> f, BCDesc<float> bc, Phys<float> k, int pull,
             float rho, u[3];
             collide_shifted(k, pop, F, rho, u);
             const unsigned vs4 = (unsigned)f.vs * 4u, c4 = cb * 4u;
+            if (!MASKED || wmask.writes(pb)) {
 #pragma unroll
-            for (int j = 0; j < kQ; ++j) sto_hint<NT>(f.gn, j * vs4 + c4, pop[j]);
+                for (int j = 0; j < kQ; ++j) sto_hint<NT>(f.gn, j * vs4 + c4, pop[j]);
 #pragma unroll
-            for (int m = 0; m < 3; ++m) sto_hint<NT>(f.un[m], c4, u[m]);
+                for (int m = 0; m < 3; ++m) sto_hint<NT>(f.un[m], c4, u[m]);
 #pragma unroll
-            for (int m = 0; m < 5; ++m) sto_hint<NT>(f.qn[m], c4, qc[m]);
-            if (aux) {
-                sto(f.rho, c4, rho);
-                sto(f.F[0], c4, F[0]);
-                sto(f.F[1], c4, F[1]);
-                sto(f.F[2], c4, F[2]);
+                for (int m = 0; m < 5; ++m) sto_hint<NT>(f.qn[m], c4, qc[m]);
+                if (aux) {
+                    sto(f.rho, c4, rho);
+                    sto(f.F[0], c4, F[0]);
+                    sto(f.F[1], c4, F[1]);
+                    sto(f.F[2], c4, F[2]);
+                }
             }
         }
         // Plane pb becomes the lower z neighbour of the next plane.
@@ -832,6 +904,42 @@ constexpr int kNumTiles = sizeof(kTiles) / sizeof(kTiles[0]);
 int fused_tile_count() { return kNumTiles; }
 const char* fused_tile_name(int tile) { return (tile >= 0 && tile < kNumTiles) ? kTiles[tile].name : "?"; }
 
+// Shell launches of a domain with walls. Shell tiles are 4 cells across the
+// wall-normal axis (x, y) or 2 planes (z), so little of the work they
+// recompute lies outside the shell: one-wave 4x16 / 16x4 column tiles
+// marching 8 planes for the x and y shells, 16x16 tiles of 2 planes for z.
+template <int TX, int TY, int ZC>
+static void launch_shell(const Fields<float>& f, const BCDesc<float>& bc, const Phys<float>& k, int pull, int aux,
+                         int axis, hipStream_t s) {
+    const int T[3] = {TX, TY, ZC};
+    ShellTiles st{};
+    st.axis = axis;
+    for (int a = 0; a < 3; ++a) st.nt[a] = (bc.n[a] + T[a] - 1) / T[a];
+    const int n = bc.n[axis];
+    auto open = [&](int face) { return !(bc.qkind[face] == QB_PERIODIC && bc.ukind[face] == UB_PERIODIC); };
+    auto add = [&](int c) {
+        const int t = c / T[axis];
+        for (int i = 0; i < st.cnt; ++i)
+            if (st.idx[i] == t) return;
+        st.idx[st.cnt++] = t;
+    };
+    if (open(2 * axis))
+        for (int c = 0; c < std::min(2, n); ++c) add(c);
+    if (open(2 * axis + 1))
+        for (int c = std::max(n - 2, 0); c < n; ++c) add(c);
+    if (st.cnt == 0) return;
+    const int a1 = (axis + 1) % 3, a2 = (axis + 2) % 3;
+    const int blocks = st.cnt * st.nt[a1] * st.nt[a2];
+    hipLaunchKernelGGL((fused_shell_kernel<TX, TY, ZC>), dim3(blocks), dim3(TX * TY), 0, s, f, bc, k, pull, aux, st);
+}
+
+static void launch_shells(const Fields<float>& f, const BCDesc<float>& bc, const Phys<float>& k, int pull, int aux,
+                          hipStream_t s) {
+    launch_shell<4, 16, 8>(f, bc, k, pull, aux, 0, s);
+    launch_shell<16, 4, 8>(f, bc, k, pull, aux, 1, s);
+    launch_shell<16, 16, 2>(f, bc, k, pull, aux, 2, s);
+}
+
 template <int KIND, int TX, int TY, int ZC>
 static void launch_tile(const Fields<float>& f, const BCDesc<float>& bc, const Phys<float>& k, int pull, int aux,
                         hipStream_t s) {
@@ -842,24 +950,34 @@ static void launch_tile(const Fields<float>& f, const BCDesc<float>& bc, const P
     for (int face = 0; face < 6; ++face)
         periodic = periodic && bc.qkind[face] == QB_PERIODIC && bc.ukind[face] == UB_PERIODIC;
     const bool up = k.advection != 0;
-#define LBM_FUSED_PF(P, U, N)                                                                              \
-    hipLaunchKernelGGL((fused_step_kernel_pf<TX, TY, ZC, P, 2, U, N>), dim3(grid), dim3(TX * TY), 0, s, f, bc, k, \
-                       pull, aux, ntx, nty, nzc)
+#define LBM_FUSED_PF(M, U, N)                                                                              \
+    hipLaunchKernelGGL((fused_step_kernel_pf<TX, TY, ZC, M, 2, U, N>), dim3(grid), dim3(TX * TY), 0, s, f, bc, k, \
+                       pull, aux, ntx, nty, nzc, M ? 1 : 0)
+    // Periodic domains run one launch. On domains with walls the periodic
+    // specialisation of the chosen pipeline runs over every tile and writes
+    // the cells two or more cells away from every open face; the first
+    // pipeline with full wall handling then writes the shell of each axis
+    // with an open face, from tiles that are thin across that axis. The
+    // launches write disjoint cells and read only the previous state.
+    const int mode = periodic ? 0 : 1;
     if constexpr (KIND == 0) {
-        if (periodic)
-            hipLaunchKernelGGL((fused_step_kernel<TX, TY, ZC, true>), dim3(grid), dim3(TX * TY), 0, s, f, bc, k,
-                               pull, aux, ntx, nty, nzc);
-        else
-            hipLaunchKernelGGL((fused_step_kernel<TX, TY, ZC, false>), dim3(grid), dim3(TX * TY), 0, s, f, bc, k,
-                               pull, aux, ntx, nty, nzc);
+        hipLaunchKernelGGL((fused_step_kernel<TX, TY, ZC, true>), dim3(grid), dim3(TX * TY), 0, s, f, bc, k, pull,
+                           aux, ntx, nty, nzc, mode);
     } else if constexpr (KIND == 1) {
-        if (periodic) { if (up) LBM_FUSED_PF(true, true, false); else LBM_FUSED_PF(true, false, false); }
-        else { if (up) LBM_FUSED_PF(false, true, false); else LBM_FUSED_PF(false, false, false); }
+        if (periodic) {
+            if (up) LBM_FUSED_PF(false, true, false); else LBM_FUSED_PF(false, false, false);
+        } else {
+            if (up) LBM_FUSED_PF(true, true, false); else LBM_FUSED_PF(true, false, false);
+        }
     } else {
-        if (periodic) { if (up) LBM_FUSED_PF(true, true, true); else LBM_FUSED_PF(true, false, true); }
-        else { if (up) LBM_FUSED_PF(false, true, true); else LBM_FUSED_PF(false, false, true); }
+        if (periodic) {
+            if (up) LBM_FUSED_PF(false, true, true); else LBM_FUSED_PF(false, false, true);
+        } else {
+            if (up) LBM_FUSED_PF(true, true, true); else LBM_FUSED_PF(true, false, true);
+        }
     }
 #undef LBM_FUSED_PF
+    if (!periodic) launch_shells(f, bc, k, pull, aux, s);
 }
 
 void launch_fused_step_f32(const Fields<float>& f, const BCDesc<float>& bc, const Phys<float>& k, int tile,

@@ -17,7 +17,12 @@ pytestmark = pytest.mark.gpu
 
 CASES = [("FullyPeriodic", (37, 29, 23), abi.ADV_UPWIND), ("Channel", (40, 20, 12), abi.ADV_CENTRED),
          ("SlitFreeSlip", (24, 30, 40), abi.ADV_CENTRED), ("MovingLidZ", (20, 18, 35), abi.ADV_UPWIND),
-         ("AnchoredZ", (16, 16, 16), abi.ADV_CENTRED), ("SpecularXNoSlipY", (19, 21, 9), abi.ADV_UPWIND)]
+         ("AnchoredZ", (16, 16, 16), abi.ADV_CENTRED), ("SpecularXNoSlipY", (19, 21, 9), abi.ADV_UPWIND),
+         # cells far from the walls (periodic specialisation) next to the wall shells
+         ("SlitNoSlip", (80, 36, 24), abi.ADV_CENTRED), ("Channel", (72, 44, 20), abi.ADV_UPWIND),
+         ("MovingLidZ", (24, 20, 48), abi.ADV_UPWIND),
+         # overlapping shells: every cell within two cells of a wall
+         ("Channel", (3, 5, 10), abi.ADV_CENTRED)]
 
 
 def rel(a, b):
