@@ -29,8 +29,11 @@ namespace lbm {
 
 // Map a coordinate of the extended tile into the domain: periodic side wraps,
 // any other side clamps (the rule of nb_index for arbitrary distance).
-// The extended tile reaches at most two cells past a face, so a periodic wrap
-// needs at most two additions or subtractions of n (also for n = 1).
+// map_coord serves z planes, which lie at most two cells past a face (one
+// addition or subtraction of n, or two for n = 1). map_col serves the x and y
+// columns, fixed per thread: the inactive columns of a tile wider than the
+// domain can lie further out and wrap by a remainder, so that every load
+// stays inside the domain.
 __device__ __forceinline__ int map_coord(int t, int n, bool lo_per, bool hi_per) {
     if (t < 0) {
         if (!lo_per) return 0;
@@ -41,6 +44,19 @@ __device__ __forceinline__ int map_coord(int t, int n, bool lo_per, bool hi_per)
         if (!hi_per) return n - 1;
         t -= n;
         return t >= n ? t - n : t;
+    }
+    return t;
+}
+__device__ __forceinline__ int map_col(int t, int n, bool lo_per, bool hi_per) {
+    if (t < 0) {
+        if (!lo_per) return 0;
+        t += n;
+        return t >= 0 ? t : (n - 1) - ((-t - 1) % n);
+    }
+    if (t >= n) {
+        if (!hi_per) return n - 1;
+        t -= n;
+        return t < n ? t : t % n;
     }
     return t;
 }
@@ -332,11 +348,11 @@ __device__ __forceinline__ float stage_value(const StageA& s, int m) {
 // 1 + axis.
 //
 // Write modes of a launch: 0 every cell, 1 class 0 only, 2 + a the shell of
-// axis a. The class is formed once per column and once per plane with
-// integer arithmetic, with no branches: boolean tests of the face tags
-// evaluated inside a lane-divergent short circuit were seen to be reused by
-// the compiler, as lane masks of that branch, in the ghost tests of the
-// force stage (q_pair), which then took a periodic face for a wall.
+// axis a. A mode gates the stores only; the force stage runs for every
+// active cell as in an unmasked launch. (With the whole force stage under the
+// mask, the build for gfx950 applied the Neumann ghost across a periodic z
+// face in the lanes that wrote, at the first and last planes; gating the
+// stores alone gives the reference values.)
 struct CellMask {
     int mode, col, zlo, zhi, nz;
     __device__ __forceinline__ bool writes(int z) const {
@@ -405,16 +421,16 @@ __device__ __forceinline__ void fused_body(const Fields<float>& f, const BCDesc<
     if (has_outer) ring_col(orank, G::IX, G::IY, oi, oj);
 
     // Domain columns (Q tags / velocity tags) of the owned columns.
-    const int qx0 = map_coord(gx, nx, ax[0].qlo, ax[0].qhi), qy0 = map_coord(gy, ny, ax[1].qlo, ax[1].qhi);
-    const int ux0 = map_coord(gx, nx, ax[0].ulo, ax[0].uhi), uy0 = map_coord(gy, ny, ax[1].ulo, ax[1].uhi);
+    const int qx0 = map_col(gx, nx, ax[0].qlo, ax[0].qhi), qy0 = map_col(gy, ny, ax[1].qlo, ax[1].qhi);
+    const int ux0 = map_col(gx, nx, ax[0].ulo, ax[0].uhi), uy0 = map_col(gy, ny, ax[1].ulo, ax[1].uhi);
     const int rgx = x0 - 1 + ra, rgy = y0 - 1 + rb;
-    const int qxr = map_coord(rgx, nx, ax[0].qlo, ax[0].qhi), qyr = map_coord(rgy, ny, ax[1].qlo, ax[1].qhi);
-    const int uxr = map_coord(rgx, nx, ax[0].ulo, ax[0].uhi), uyr = map_coord(rgy, ny, ax[1].ulo, ax[1].uhi);
+    const int qxr = map_col(rgx, nx, ax[0].qlo, ax[0].qhi), qyr = map_col(rgy, ny, ax[1].qlo, ax[1].qhi);
+    const int uxr = map_col(rgx, nx, ax[0].ulo, ax[0].uhi), uyr = map_col(rgy, ny, ax[1].ulo, ax[1].uhi);
     int oqc = 0, ouc = 0;
     if (has_outer) {
         const int ogx = x0 - 2 + oi, ogy = y0 - 2 + oj;
-        oqc = map_coord(ogy, ny, ax[1].qlo, ax[1].qhi) * nx + map_coord(ogx, nx, ax[0].qlo, ax[0].qhi);
-        ouc = map_coord(ogy, ny, ax[1].ulo, ax[1].uhi) * nx + map_coord(ogx, nx, ax[0].ulo, ax[0].uhi);
+        oqc = map_col(ogy, ny, ax[1].qlo, ax[1].qhi) * nx + map_col(ogx, nx, ax[0].qlo, ax[0].qhi);
+        ouc = map_col(ogy, ny, ax[1].ulo, ax[1].uhi) * nx + map_col(ogx, nx, ax[0].ulo, ax[0].uhi);
     }
     const int qc0 = qy0 * nx + qx0, uc0 = uy0 * nx + ux0;
     const int qcr = qyr * nx + qxr, ucr = uyr * nx + uxr;
@@ -653,12 +669,14 @@ fused_shell_kernel(Fields<float> f, BCDesc<float> bc, Phys<float> k, int pull, i
 // is published after a barrier), and with every global load issued one z
 // plane ahead of its use: the pull gather of plane pb and the input planes
 // pa + 2 are in flight while the Q update of plane pa computes. Only the
-// periodic specialisation is built; on a domain with walls (MASKED) it
-// writes the class-0 cells (write mode 1) and the shell launches do the rest.
+// periodic specialisation is built. On a domain with walls (MASKED) its
+// values for the shell cells are wrong and the shell launches that follow
+// overwrite them; rho and F, of which rho is also read by the shells, are
+// written for class-0 cells only.
 template <int TX, int TY, int ZC, bool MASKED, int WPE, bool UPWIND, bool NT>
 __global__ void __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(WPE, WPE)))
 fused_step_kernel_pf(Fields<float> f, BCDesc<float> bc, Phys<float> k, int pull, int aux, int ntx, int nty,
-                     int nzc, int mode) {
+                     int nzc) {
     using G = TileGeom<TX, TY>;
     constexpr bool PERIODIC = true;
     __shared__ float sIn[kNumIn][G::NI];           // input plane pa, extended region
@@ -689,7 +707,6 @@ fused_step_kernel_pf(Fields<float> f, BCDesc<float> bc, Phys<float> k, int pull,
     const int acell = ib * G::AX + ia;
     const int gx = x0 + ia - 1, gy = y0 + ib - 1;
     const bool active = gx < nx && gy < ny;
-    const CellMask wmask = cell_mask(bc, mode, gx, gy);
     const bool has_ring = t < G::NR;
     int ra = 0, rb = 0;
     if (has_ring) ring_col(t, G::AX, G::AY, ra, rb);
@@ -698,16 +715,16 @@ fused_step_kernel_pf(Fields<float> f, BCDesc<float> bc, Phys<float> k, int pull,
     int oi = 0, oj = 0;
     if (has_outer) ring_col(orank, G::IX, G::IY, oi, oj);
 
-    const int qx0 = map_coord(gx, nx, ax[0].qlo, ax[0].qhi), qy0 = map_coord(gy, ny, ax[1].qlo, ax[1].qhi);
-    const int ux0 = map_coord(gx, nx, ax[0].ulo, ax[0].uhi), uy0 = map_coord(gy, ny, ax[1].ulo, ax[1].uhi);
+    const int qx0 = map_col(gx, nx, ax[0].qlo, ax[0].qhi), qy0 = map_col(gy, ny, ax[1].qlo, ax[1].qhi);
+    const int ux0 = map_col(gx, nx, ax[0].ulo, ax[0].uhi), uy0 = map_col(gy, ny, ax[1].ulo, ax[1].uhi);
     const int rgx = x0 - 1 + ra, rgy = y0 - 1 + rb;
-    const int qxr = map_coord(rgx, nx, ax[0].qlo, ax[0].qhi), qyr = map_coord(rgy, ny, ax[1].qlo, ax[1].qhi);
-    const int uxr = map_coord(rgx, nx, ax[0].ulo, ax[0].uhi), uyr = map_coord(rgy, ny, ax[1].ulo, ax[1].uhi);
+    const int qxr = map_col(rgx, nx, ax[0].qlo, ax[0].qhi), qyr = map_col(rgy, ny, ax[1].qlo, ax[1].qhi);
+    const int uxr = map_col(rgx, nx, ax[0].ulo, ax[0].uhi), uyr = map_col(rgy, ny, ax[1].ulo, ax[1].uhi);
     int oqc = 0, ouc = 0;
     if (has_outer) {
         const int ogx = x0 - 2 + oi, ogy = y0 - 2 + oj;
-        oqc = map_coord(ogy, ny, ax[1].qlo, ax[1].qhi) * nx + map_coord(ogx, nx, ax[0].qlo, ax[0].qhi);
-        ouc = map_coord(ogy, ny, ax[1].ulo, ax[1].uhi) * nx + map_coord(ogx, nx, ax[0].ulo, ax[0].uhi);
+        oqc = map_col(ogy, ny, ax[1].qlo, ax[1].qhi) * nx + map_col(ogx, nx, ax[0].qlo, ax[0].qhi);
+        ouc = map_col(ogy, ny, ax[1].ulo, ax[1].uhi) * nx + map_col(ogx, nx, ax[0].ulo, ax[0].uhi);
     }
     const int qc0 = qy0 * nx + qx0, uc0 = uy0 * nx + ux0;
     const int qcr = qyr * nx + qxr, ucr = uyr * nx + uxr;
@@ -832,19 +849,19 @@ fused_step_kernel_pf(Fields<float> f, BCDesc<float> bc, Phys<float> k, int pull,
             float rho, u[3];
             collide_shifted(k, pop, F, rho, u);
             const unsigned vs4 = (unsigned)f.vs * 4u, c4 = cb * 4u;
-            if (!MASKED || wmask.writes(pb)) {
 #pragma unroll
-                for (int j = 0; j < kQ; ++j) sto_hint<NT>(f.gn, j * vs4 + c4, pop[j]);
+            for (int j = 0; j < kQ; ++j) sto_hint<NT>(f.gn, j * vs4 + c4, pop[j]);
 #pragma unroll
-                for (int m = 0; m < 3; ++m) sto_hint<NT>(f.un[m], c4, u[m]);
+            for (int m = 0; m < 3; ++m) sto_hint<NT>(f.un[m], c4, u[m]);
 #pragma unroll
-                for (int m = 0; m < 5; ++m) sto_hint<NT>(f.qn[m], c4, qc[m]);
-                if (aux) {
-                    sto(f.rho, c4, rho);
-                    sto(f.F[0], c4, F[0]);
-                    sto(f.F[1], c4, F[1]);
-                    sto(f.F[2], c4, F[2]);
-                }
+            for (int m = 0; m < 5; ++m) sto_hint<NT>(f.qn[m], c4, qc[m]);
+            // rho is also an input (moving-wall bounce of the shell cells),
+            // so on a domain with walls only the class-0 cells write it.
+            if (aux && (!MASKED || cell_mask(bc, 1, gx, gy).writes(pb))) {
+                sto(f.rho, c4, rho);
+                sto(f.F[0], c4, F[0]);
+                sto(f.F[1], c4, F[1]);
+                sto(f.F[2], c4, F[2]);
             }
         }
         // Plane pb becomes the lower z neighbour of the next plane.
@@ -906,8 +923,10 @@ const char* fused_tile_name(int tile) { return (tile >= 0 && tile < kNumTiles) ?
 
 // Shell launches of a domain with walls. Shell tiles are 4 cells across the
 // wall-normal axis (x, y) or 2 planes (z), so little of the work they
-// recompute lies outside the shell: one-wave 4x16 / 16x4 column tiles
-// marching 8 planes for the x and y shells, 16x16 tiles of 2 planes for z.
+// recompute lies outside the shell: 4x64 / 64x4 column tiles marching 4
+// planes for the x and y shells, 32x8 tiles of 2 planes for z. (At 256^3 the
+// x shell costs 0.27-0.40 ms over the tile shapes 4x16..4x64 and chunks of
+// 4..16 planes; 4x64x4 was the fastest.)
 template <int TX, int TY, int ZC>
 static void launch_shell(const Fields<float>& f, const BCDesc<float>& bc, const Phys<float>& k, int pull, int aux,
                          int axis, hipStream_t s) {
@@ -935,9 +954,9 @@ static void launch_shell(const Fields<float>& f, const BCDesc<float>& bc, const 
 
 static void launch_shells(const Fields<float>& f, const BCDesc<float>& bc, const Phys<float>& k, int pull, int aux,
                           hipStream_t s) {
-    launch_shell<4, 16, 8>(f, bc, k, pull, aux, 0, s);
-    launch_shell<16, 4, 8>(f, bc, k, pull, aux, 1, s);
-    launch_shell<16, 16, 2>(f, bc, k, pull, aux, 2, s);
+    launch_shell<4, 64, 4>(f, bc, k, pull, aux, 0, s);
+    launch_shell<64, 4, 4>(f, bc, k, pull, aux, 1, s);
+    launch_shell<32, 8, 2>(f, bc, k, pull, aux, 2, s);
 }
 
 template <int KIND, int TX, int TY, int ZC>
@@ -952,13 +971,15 @@ static void launch_tile(const Fields<float>& f, const BCDesc<float>& bc, const P
     const bool up = k.advection != 0;
 #define LBM_FUSED_PF(M, U, N)                                                                              \
     hipLaunchKernelGGL((fused_step_kernel_pf<TX, TY, ZC, M, 2, U, N>), dim3(grid), dim3(TX * TY), 0, s, f, bc, k, \
-                       pull, aux, ntx, nty, nzc, M ? 1 : 0)
+                       pull, aux, ntx, nty, nzc)
     // Periodic domains run one launch. On domains with walls the periodic
-    // specialisation of the chosen pipeline runs over every tile and writes
-    // the cells two or more cells away from every open face; the first
-    // pipeline with full wall handling then writes the shell of each axis
-    // with an open face, from tiles that are thin across that axis. The
-    // launches write disjoint cells and read only the previous state.
+    // specialisation of the chosen pipeline runs over every tile; its values
+    // are exact for the cells two or more cells away from every open face.
+    // The first pipeline with full wall handling then writes the shell of
+    // each axis with an open face, from tiles that are thin across that
+    // axis, over the prefetch pipeline's shell values. Each launch reads the
+    // previous state only; rho, which the shells read for moving walls, is
+    // written by the owner of each cell alone.
     const int mode = periodic ? 0 : 1;
     if constexpr (KIND == 0) {
         hipLaunchKernelGGL((fused_step_kernel<TX, TY, ZC, true>), dim3(grid), dim3(TX * TY), 0, s, f, bc, k, pull,
