@@ -249,7 +249,10 @@ __device__ __forceinline__ void stage_a_lean(const BCDesc<float>& bc, const Phys
             case 3: Q2 = Qxy * Qxy + Qyy * Qyy + Qyz * Qyz - third * TrQ2; break;
             default: Q2 = Qxy * Qxz - Qyz * Qxx; break;
         }
-        H[m] = k.L * d.lap - ld * c[m] - k.B * Q2;
+        // With upwind advection the per-axis second differences exist anyway;
+        // their sum is the same Laplacian with three fewer additions.
+        const float lap = UPWIND ? (d.d2[0] + d.d2[1]) + d.d2[2] : d.lap;
+        H[m] = k.L * lap - ld * c[m] - k.B * Q2;
         adv[m] = -(adv_axis(UPWIND, u0, d.d[0], d.d2[0]) + adv_axis(UPWIND, u1, d.d[1], d.d2[1])
                    + adv_axis(UPWIND, u2, d.d[2], d.d2[2]));
         // -H : grad_a Q over the nine index pairs (Qzz = -Qxx - Qyy, Hzz = -Hxx - Hyy).

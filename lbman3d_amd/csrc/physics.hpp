@@ -243,20 +243,29 @@ __device__ __forceinline__ void collide_plain(const Phys<T>& k, T f[kQ], const T
 // rounded: rho = rho0 + sum d_i, sum e_i f_i = sum e_i d_i, and since
 // omega + omega' = 1,
 //   d*_i = omega d_i + omega' w_i (drho + rho (3 ue + 4.5 ue^2 - 1.5 u^2)) + DT F_i.
+// The directions are taken in opposite pairs (e_o = -e_i, w_o = w_i): with
+// ue = e_i.u and eF = e_i.F the update splits into a part even in e_i,
+//   S = w (omega' (drho - 1.5 rho u^2 + 4.5 rho ue^2) + DT omega_f (-3 u.F + 9 ue eF)),
+// and an odd part A = 3 w (omega' rho ue + DT omega_f eF), so that
+// d*_i = omega d_i + S + A and d*_o = omega d_o + S - A.
 template <typename T>
 __device__ __forceinline__ void collide_shifted(const Phys<T>& k, T d[kQ], const T F[3], T& rho_out,
                                                 T u_out[3]) {
-    T drho = T(0.0), mx = T(0.0), my = T(0.0), mz = T(0.0);
-#pragma unroll
-    for (int i = 0; i < kQ; ++i) {
-        drho += d[i];
-        if (cx(i)) mx += T(cx(i)) * d[i];
-        if (cy(i)) my += T(cy(i)) * d[i];
-        if (cz(i)) mz += T(cz(i)) * d[i];
-    }
+    // Pair differences p = d_i - d_o carry the momentum; e_i of the first of
+    // each pair: 1 (1,0,0), 2 (0,1,0), 5 (0,0,1), 7 (1,1,1), 8 (-1,1,1),
+    // 9 (-1,-1,1), 10 (1,-1,1).
+    const T p1 = d[1] - d[3], p2 = d[2] - d[4], p5 = d[5] - d[6];
+    const T p7 = d[7] - d[13], p8 = d[8] - d[14], p9 = d[9] - d[11], p10 = d[10] - d[12];
+    const T drho = ((d[0] + (d[1] + d[3])) + ((d[2] + d[4]) + (d[5] + d[6])))
+                 + (((d[7] + d[13]) + (d[8] + d[14])) + ((d[9] + d[11]) + (d[10] + d[12])));
+    T mx = (p1 + p7) + (p10 - (p8 + p9));
+    T my = (p2 + p7) + (p8 - (p9 + p10));
+    T mz = (p5 + p7) + ((p8 + p9) + p10);
     const T rho = k.rho0 + drho;
     const T hdt = T(0.5) * k.dt;
-    const T inv = T(1.0) / rho;
+    T inv;
+    if constexpr (sizeof(T) == 4) inv = __builtin_amdgcn_rcpf(rho);   // v_rcp_f32, 1 ulp
+    else inv = T(1.0) / rho;
     mx = (mx + hdt * F[0]) * inv;
     my = (my + hdt * F[1]) * inv;
     mz = (mz + hdt * F[2]) * inv;
@@ -266,19 +275,26 @@ __device__ __forceinline__ void collide_shifted(const Phys<T>& k, T d[kQ], const
     u_out[2] = mz;
     const T uF = mx * F[0] + my * F[1] + mz * F[2];
     const T u2 = mx * mx + my * my + mz * mz;
-    const T base = drho - T(1.5) * rho * u2;    // w-independent part of (feq - w rho0)/w
-    const T frc0 = T(-3.0) * uF;                // direction-independent part of the Guo bracket
-#pragma unroll
-    for (int i = 0; i < kQ; ++i) {
-        const T wi = T(wgt(i));
-        T ue = T(0.0), eF = T(0.0);
-        if (cx(i)) { ue += T(cx(i)) * mx; eF += T(cx(i)) * F[0]; }
-        if (cy(i)) { ue += T(cy(i)) * my; eF += T(cy(i)) * F[1]; }
-        if (cz(i)) { ue += T(cz(i)) * mz; eF += T(cz(i)) * F[2]; }
-        const T dfeq = wi * (base + rho * ue * (T(3.0) + T(4.5) * ue));
-        const T frc = k.omega_f * wi * (T(3.0) * eF + frc0 + T(9.0) * ue * eF);
-        d[i] = k.omega * d[i] + k.omega_p * dfeq + k.dt * frc;
-    }
+    const T cf = k.dt * k.omega_f;
+    // Direction-independent part of S / w, and the factors of the two weight classes.
+    const T s0 = k.omega_p * (drho - T(1.5) * rho * u2) - T(3.0) * cf * uF;
+    const T r45 = T(4.5) * k.omega_p * rho, c9 = T(9.0) * cf;
+    const T a1 = T(3.0) * k.omega_p * rho, a2 = T(3.0) * cf;
+    d[0] = k.omega * d[0] + T(wgt(0)) * s0;
+    auto pair = [&](int i, int o, T ue, T eF) {
+        const T w = T(wgt(i));
+        const T S = w * (s0 + ue * (r45 * ue + c9 * eF));
+        const T A = w * (a1 * ue + a2 * eF);
+        d[i] = k.omega * d[i] + (S + A);
+        d[o] = k.omega * d[o] + (S - A);
+    };
+    pair(1, 3, mx, F[0]);
+    pair(2, 4, my, F[1]);
+    pair(5, 6, mz, F[2]);
+    pair(7, 13, mx + my + mz, F[0] + F[1] + F[2]);
+    pair(8, 14, my + mz - mx, F[1] + F[2] - F[0]);
+    pair(9, 11, mz - mx - my, F[2] - F[0] - F[1]);
+    pair(10, 12, mx - my + mz, F[0] - F[1] + F[2]);
 }
 
 }  // namespace lbm
