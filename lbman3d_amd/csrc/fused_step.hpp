@@ -895,11 +895,235 @@ fused_step_kernel_pf(Fields<float> f, BCDesc<float> bc, Phys<float> k, int pull,
     }
 }
 
+// Divergence form of the body force. With P = -alpha Q^{n+1} + (Sigma +/- Tau)
+// in the sign pattern of body_force (src/qtensor_solver.tpp:154-217,
+// src/boundary_handler.h:335-392), F_a = d_b P_ab - mu u_a + F_E,a, and a
+// neighbour along axis b contributes only the column (P_xb, P_yb, P_zb):
+//   X set (x neighbours): -aQxx + Sxx, -aQxy + Sxy - Txy, -aQxz + Sxz - Txz;
+//   Y set (y neighbours): -aQxy + Sxy + Txy, -aQyy + Syy, -aQyz + Syz - Tyz;
+//   Z set (z neighbours): -aQxz + Sxz + Txz, -aQyz + Syz + Tyz, a(Qxx + Qyy) - Sxx - Syy.
+// (Szz = -Sxx - Syy and Qzz = -Qxx - Qyy.)
+__device__ __forceinline__ void p_sets(float alpha, const StageA& a, float pxy[6], float pz[3]) {
+    const float* q = a.qn;
+    const float* sg = a.sg;
+    const float* ta = a.ta;
+    pxy[0] = sg[0] - alpha * q[0];
+    pxy[1] = (sg[1] - ta[0]) - alpha * q[1];
+    pxy[2] = (sg[2] - ta[1]) - alpha * q[2];
+    pxy[3] = (sg[1] + ta[0]) - alpha * q[1];
+    pxy[4] = sg[3] - alpha * q[3];
+    pxy[5] = (sg[4] - ta[2]) - alpha * q[4];
+    pz[0] = (sg[2] + ta[1]) - alpha * q[2];
+    pz[1] = (sg[4] + ta[2]) - alpha * q[4];
+    pz[2] = alpha * (q[0] + q[3]) - (sg[0] + sg[3]);
+}
+
+// Third pipeline (periodic specialisation, like the second): the force
+// stage reads only the divergence-form sets above (6 values per column in
+// LDS instead of 13; the z set of the own column stays in registers), and the
+// input plane and the sets are double-buffered in LDS, so one barrier per
+// plane separates the writes of a plane from its reads. Per iteration pa:
+//   before the barrier: publish the sets of plane pa - 1, stage input plane pa,
+//     store ring inputs of plane pa + 1, issue the loads of planes pa + 1/pa + 2
+//     and the pull gather of plane pa - 1;
+//   after it: Q update at plane pa, then force + collide-stream at plane pa - 1.
+// A wave can run at most one iteration ahead of the slowest (the next
+// barrier), which the two LDS buffers and the four ring-input slots cover.
+template <int TX, int TY, int ZC, bool MASKED, int WPE, bool UPWIND, bool NT>
+__global__ void __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(WPE, WPE)))
+fused_step_kernel_p2(Fields<float> f, BCDesc<float> bc, Phys<float> k, int pull, int aux, int ntx, int nty,
+                     int nzc) {
+    using G = TileGeom<TX, TY>;
+    __shared__ float sIn[2][kNumIn][G::NI];        // input plane, by parity
+    __shared__ float sP[2][6][G::NA];              // X and Y sets, by plane parity
+    __shared__ float sRing[4][kNumIn][G::NR];      // ring-column inputs, planes mod 4
+
+    const int nblk = ntx * nty * nzc;
+    const int per_xcd = (nblk + 7) / 8;
+    const int L = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
+    if (L >= nblk) return;
+    const int tx = L % ntx, ty = (L / ntx) % nty, tz = L / (ntx * nty);
+    const int nx = bc.n[0], ny = bc.n[1], nz = bc.n[2];
+    const int x0 = tx * TX, y0 = ty * TY;
+    const int z0 = tz * ZC, z1 = min(z0 + ZC, nz);
+    const unsigned plane = (unsigned)nx * ny;
+
+    const int t = threadIdx.x;
+    const int ia = t % TX + 1, ib = t / TX + 1;
+    const int acell = ib * G::AX + ia;
+    const int gx = x0 + ia - 1, gy = y0 + ib - 1;
+    const bool active = gx < nx && gy < ny;
+    const bool has_ring = t < G::NR;
+    int ra = 0, rb = 0;
+    if (has_ring) ring_col(t, G::AX, G::AY, ra, rb);
+    const int orank = t - (G::NT - G::NO);
+    const bool has_outer = orank >= 0;
+    int oi = 0, oj = 0;
+    if (has_outer) ring_col(orank, G::IX, G::IY, oi, oj);
+
+    const int qx0 = map_col(gx, nx, true, true), qy0 = map_col(gy, ny, true, true);
+    const int qxr = map_col(x0 - 1 + ra, nx, true, true), qyr = map_col(y0 - 1 + rb, ny, true, true);
+    int oc = 0;
+    if (has_outer) oc = map_col(y0 - 2 + oj, ny, true, true) * nx + map_col(x0 - 2 + oi, nx, true, true);
+    const int c0 = qy0 * nx + qx0, cr = qyr * nx + qxr;
+    auto zplane = [&](int z) { return (unsigned)map_coord(z, nz, true, true) * plane; };
+    const int cc_int = (ib + 1) * G::IX + ia + 1;
+    const int cc_ring = (rb + 1) * G::IX + ra + 1;
+    const int cc_out = oj * G::IX + oi;
+    const int rcell = rb * G::AX + ra;
+    const int qpos_i[3] = {qx0, qy0, 0}, qpos_r[3] = {qxr, qyr, 0};
+
+    // Prologue: interior planes z0-2 .. z0 (registers), ring planes z0-2 and
+    // z0-1 (LDS) and z0 (registers, stored in the first iteration), outer
+    // plane z0-1.
+    float hin[3][kNumIn], rcur[kNumIn], ov[kNumIn];
+    load_column<TX, TY>(f, c0, c0, zplane(z0 - 2), zplane(z0 - 2), hin[0]);
+    load_column<TX, TY>(f, c0, c0, zplane(z0 - 1), zplane(z0 - 1), hin[1]);
+    load_column<TX, TY>(f, c0, c0, zplane(z0), zplane(z0), hin[2]);
+    if (has_ring) {
+#pragma unroll 1
+        for (int z = z0 - 2; z <= z0 - 1; ++z) {
+            float v[kNumIn];
+            load_column<TX, TY>(f, cr, cr, zplane(z), zplane(z), v);
+#pragma unroll
+            for (int m = 0; m < kNumIn; ++m) sRing[z & 3][m][t] = v[m];
+        }
+        load_column<TX, TY>(f, cr, cr, zplane(z0), zplane(z0), rcur);
+    }
+    if (has_outer) load_column<TX, TY>(f, oc, oc, zplane(z0 - 1), zplane(z0 - 1), ov);
+
+    // Carried across iterations (interior column): sets of plane pa - 1 to
+    // publish, z sets of planes pa - 2 and pa - 1, Q^{n+1} and Ericksen force
+    // of plane pa - 1; ring column: sets of plane pa - 1.
+    float pxy_prev[6] = {0, 0, 0, 0, 0, 0}, pr_prev[6] = {0, 0, 0, 0, 0, 0};
+    float zm2[3] = {0, 0, 0}, zm1[3] = {0, 0, 0};
+    float qn_prev[5] = {0, 0, 0, 0, 0}, fe_prev[3] = {0, 0, 0};
+
+    for (int pa = z0 - 1; pa <= z1; ++pa) {
+        const int pb = pa - 1;
+        const int bi = pa & 1;
+        const bool do_b = pb >= z0 && active;
+        // ---- before the barrier: loads in flight, publish plane pa - 1, stage plane pa ----
+        float hnext[kNumIn], rnext[kNumIn], onext[kNumIn], pop[kQ];
+        load_column<TX, TY>(f, c0, c0, zplane(pa + 2), zplane(pa + 2), hnext);
+        if (has_ring) load_column<TX, TY>(f, cr, cr, zplane(pa + 2), zplane(pa + 2), rnext);
+        if (has_outer) load_column<TX, TY>(f, oc, oc, zplane(pa + 1), zplane(pa + 1), onext);
+        const unsigned cb = (unsigned)pb * plane + (unsigned)gy * nx + gx;
+        if (do_b) gather32<true>(f, bc, gx, gy, pb, cb, pull, pop);
+#pragma unroll
+        for (int m = 0; m < 6; ++m) sP[bi ^ 1][m][acell] = pxy_prev[m];
+#pragma unroll
+        for (int m = 0; m < kNumIn; ++m) sIn[bi][m][cc_int] = hin[1][m];
+        if (has_ring) {
+#pragma unroll
+            for (int m = 0; m < 6; ++m) sP[bi ^ 1][m][rcell] = pr_prev[m];
+#pragma unroll
+            for (int m = 0; m < kNumIn; ++m) {
+                sIn[bi][m][cc_ring] = sRing[pa & 3][m][t];
+                sRing[(pa + 1) & 3][m][t] = rcur[m];
+            }
+        }
+        if (has_outer) {
+#pragma unroll
+            for (int m = 0; m < kNumIn; ++m) sIn[bi][m][cc_out] = ov[m];
+        }
+        __syncthreads();
+
+        // ---- Q update at plane pa ----
+        const int zq = map_coord(pa, nz, true, true);
+        StageA ai;
+        float fe_new[3], z_new[3], pxy_new[6];
+        {
+            const int qpos[3] = {qpos_i[0], qpos_i[1], zq};
+            stage_a_lean<TX, TY, true, UPWIND>(bc, k, sIn[bi], ia + 1, ib + 1, hin[0], hin[1], hin[2], qpos, qpos, ai,
+                                               fe_new);
+            p_sets(k.alpha, ai, pxy_new, z_new);
+        }
+        float pr_new[6];
+        if (has_ring) {
+            const int qpos[3] = {qpos_r[0], qpos_r[1], zq};
+            float zm[kNumIn], c[kNumIn], zp[kNumIn], fer[3], zr[3];
+#pragma unroll
+            for (int m = 0; m < kNumIn; ++m) {
+                zm[m] = sRing[(pa - 1) & 3][m][t];
+                c[m] = sRing[pa & 3][m][t];
+                zp[m] = sRing[(pa + 1) & 3][m][t];
+            }
+            StageA ar;
+            stage_a_lean<TX, TY, true, UPWIND>(bc, k, sIn[bi], ra + 1, rb + 1, zm, c, zp, qpos, qpos, ar, fer);
+            p_sets(k.alpha, ar, pr_new, zr);
+        }
+
+        // ---- force + collide-stream at plane pb (sP[bi ^ 1] holds plane pb) ----
+        if (do_b) {
+            const float* P = &sP[bi ^ 1][0][0];
+            float F[3];
+            if (k.force_full) {
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    const float dx = P[a * G::NA + acell + 1] - P[a * G::NA + acell - 1];
+                    const float dy = P[(3 + a) * G::NA + acell + G::AX] - P[(3 + a) * G::NA + acell - G::AX];
+                    const float dz = z_new[a] - zm2[a];
+                    F[a] = fe_prev[a] + (0.5f * ((dx + dy) + dz) - k.mu * hin[0][5 + a]);
+                }
+            } else {
+                F[0] = fe_prev[0];
+                F[1] = fe_prev[1];
+                F[2] = fe_prev[2];
+            }
+            float rho, u[3];
+            collide_shifted(k, pop, F, rho, u);
+            const unsigned vs4 = (unsigned)f.vs * 4u, c4 = cb * 4u;
+#pragma unroll
+            for (int j = 0; j < kQ; ++j) sto_hint<NT>(f.gn, j * vs4 + c4, pop[j]);
+#pragma unroll
+            for (int m = 0; m < 3; ++m) sto_hint<NT>(f.un[m], c4, u[m]);
+#pragma unroll
+            for (int m = 0; m < 5; ++m) sto_hint<NT>(f.qn[m], c4, qn_prev[m]);
+            if (aux && (!MASKED || cell_mask(bc, 1, gx, gy).writes(pb))) {
+                sto(f.rho, c4, rho);
+                sto(f.F[0], c4, F[0]);
+                sto(f.F[1], c4, F[1]);
+                sto(f.F[2], c4, F[2]);
+            }
+        }
+
+        // ---- advance the carried values ----
+#pragma unroll
+        for (int m = 0; m < 6; ++m) pxy_prev[m] = pxy_new[m];
+        if (has_ring) {
+#pragma unroll
+            for (int m = 0; m < 6; ++m) pr_prev[m] = pr_new[m];
+#pragma unroll
+            for (int m = 0; m < kNumIn; ++m) rcur[m] = rnext[m];
+        }
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            zm2[a] = zm1[a];
+            zm1[a] = z_new[a];
+            fe_prev[a] = fe_new[a];
+        }
+#pragma unroll
+        for (int m = 0; m < 5; ++m) qn_prev[m] = ai.qn[m];
+#pragma unroll
+        for (int m = 0; m < kNumIn; ++m) {
+            hin[0][m] = hin[1][m];
+            hin[1][m] = hin[2][m];
+            hin[2][m] = hnext[m];
+        }
+        if (has_outer) {
+#pragma unroll
+            for (int m = 0; m < kNumIn; ++m) ov[m] = onext[m];
+        }
+    }
+}
+
 #ifndef LBM_FUSED_BODY_ONLY
 // ---- tile catalogue -----------------------------------------------------------
 // kind 0: first pipeline (stage-A planes double-buffered in LDS);
 // kind 1: prefetch pipeline (single stage-A plane, loads one plane ahead);
-// kind 2: prefetch pipeline with non-temporal output stores.
+// kind 2: prefetch pipeline with non-temporal output stores;
+// kind 3: third pipeline (divergence-form sets, one barrier per plane).
 struct TileShape {
     int kind, tx, ty, zc;
     const char* name;
@@ -918,6 +1142,9 @@ constexpr TileShape kTiles[] = {
     // short z chunks: enough blocks to fill the CUs on thin or small grids
     // (a 256 x 256 x 32 slab, 128^3)
     {2, 64, 4, 16, "pf-nt:64x4x16"},   {2, 32, 8, 16, "pf-nt:32x8x16"},
+    // third pipeline: divergence-form sets, one barrier per plane
+    {3, 64, 4, 128, "p2:64x4x128"},    {3, 32, 8, 128, "p2:32x8x128"},   {3, 64, 4, 64, "p2:64x4x64"},
+    {3, 64, 4, 16, "p2:64x4x16"},      {3, 32, 8, 16, "p2:32x8x16"},
 };
 constexpr int kNumTiles = sizeof(kTiles) / sizeof(kTiles[0]);
 
@@ -993,12 +1220,22 @@ static void launch_tile(const Fields<float>& f, const BCDesc<float>& bc, const P
         } else {
             if (up) LBM_FUSED_PF(true, true, false); else LBM_FUSED_PF(true, false, false);
         }
-    } else {
+    } else if constexpr (KIND == 2) {
         if (periodic) {
             if (up) LBM_FUSED_PF(false, true, true); else LBM_FUSED_PF(false, false, true);
         } else {
             if (up) LBM_FUSED_PF(true, true, true); else LBM_FUSED_PF(true, false, true);
         }
+    } else {
+#define LBM_FUSED_P2(M, U)                                                                                    \
+    hipLaunchKernelGGL((fused_step_kernel_p2<TX, TY, ZC, M, 2, U, true>), dim3(grid), dim3(TX * TY), 0, s, f, bc, \
+                       k, pull, aux, ntx, nty, nzc)
+        if (periodic) {
+            if (up) LBM_FUSED_P2(false, true); else LBM_FUSED_P2(false, false);
+        } else {
+            if (up) LBM_FUSED_P2(true, true); else LBM_FUSED_P2(true, false);
+        }
+#undef LBM_FUSED_P2
     }
 #undef LBM_FUSED_PF
     if (!periodic) launch_shells(f, bc, k, pull, aux, s);
@@ -1024,7 +1261,12 @@ void launch_fused_step_f32(const Fields<float>& f, const BCDesc<float>& bc, cons
         case 14: launch_tile<2, 32, 8, 128>(f, bc, k, pull, aux, s); break;
         case 15: launch_tile<2, 64, 8, 128>(f, bc, k, pull, aux, s); break;
         case 16: launch_tile<2, 64, 4, 16>(f, bc, k, pull, aux, s); break;
-        default: launch_tile<2, 32, 8, 16>(f, bc, k, pull, aux, s); break;
+        case 17: launch_tile<2, 32, 8, 16>(f, bc, k, pull, aux, s); break;
+        case 18: launch_tile<3, 64, 4, 128>(f, bc, k, pull, aux, s); break;
+        case 19: launch_tile<3, 32, 8, 128>(f, bc, k, pull, aux, s); break;
+        case 20: launch_tile<3, 64, 4, 64>(f, bc, k, pull, aux, s); break;
+        case 21: launch_tile<3, 64, 4, 16>(f, bc, k, pull, aux, s); break;
+        default: launch_tile<3, 32, 8, 16>(f, bc, k, pull, aux, s); break;
     }
 }
 
