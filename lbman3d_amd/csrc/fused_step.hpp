@@ -66,6 +66,11 @@ __device__ __forceinline__ int map_col(int t, int n, bool lo_per, bool hi_per) {
 __device__ __forceinline__ float ldo(const float* base, unsigned boff) {
     return *reinterpret_cast<const float*>(reinterpret_cast<const char*>(base) + boff);
 }
+// Non-temporal load: populations are read by one step only (the halo
+// re-reads of neighbouring tiles come from L2 shortly after).
+__device__ __forceinline__ float ldo_nt(const float* base, unsigned boff) {
+    return __builtin_nontemporal_load(reinterpret_cast<const float*>(reinterpret_cast<const char*>(base) + boff));
+}
 __device__ __forceinline__ void sto(float* base, unsigned boff, float v) {
     *reinterpret_cast<float*>(reinterpret_cast<char*>(base) + boff) = v;
 }
@@ -99,7 +104,7 @@ __device__ __forceinline__ void gather32(const Fields<float>& f, const BCDesc<fl
 #pragma unroll
         for (int j = 0; j < kQ; ++j) {
             const unsigned src = ((unsigned)zs[cz(j) + 1] * ny + ys[cy(j) + 1]) * nx + xs[cx(j) + 1];
-            out[j] = ldo(g, j * vs4 + src * 4u);
+            out[j] = ldo_nt(g, j * vs4 + src * 4u);
         }
         return;
     }
@@ -942,7 +947,9 @@ fused_step_kernel_p2(Fields<float> f, BCDesc<float> bc, Phys<float> k, int pull,
     const int per_xcd = (nblk + 7) / 8;
     const int L = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
     if (L >= nblk) return;
-    const int tx = L % ntx, ty = (L / ntx) % nty, tz = L / (ntx * nty);
+    // z chunks of one column are neighbours in the order (they share halo
+    // planes), then columns along x, then rows along y.
+    const int tz = L % nzc, tx = (L / nzc) % ntx, ty = L / (nzc * ntx);
     const int nx = bc.n[0], ny = bc.n[1], nz = bc.n[2];
     const int x0 = tx * TX, y0 = ty * TY;
     const int z0 = tz * ZC, z1 = min(z0 + ZC, nz);
