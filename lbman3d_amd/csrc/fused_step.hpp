@@ -66,11 +66,6 @@ __device__ __forceinline__ int map_col(int t, int n, bool lo_per, bool hi_per) {
 __device__ __forceinline__ float ldo(const float* base, unsigned boff) {
     return *reinterpret_cast<const float*>(reinterpret_cast<const char*>(base) + boff);
 }
-// Non-temporal load: populations are read by one step only (the halo
-// re-reads of neighbouring tiles come from L2 shortly after).
-__device__ __forceinline__ float ldo_nt(const float* base, unsigned boff) {
-    return __builtin_nontemporal_load(reinterpret_cast<const float*>(reinterpret_cast<const char*>(base) + boff));
-}
 __device__ __forceinline__ void sto(float* base, unsigned boff, float v) {
     *reinterpret_cast<float*>(reinterpret_cast<char*>(base) + boff) = v;
 }
@@ -104,7 +99,7 @@ __device__ __forceinline__ void gather32(const Fields<float>& f, const BCDesc<fl
 #pragma unroll
         for (int j = 0; j < kQ; ++j) {
             const unsigned src = ((unsigned)zs[cz(j) + 1] * ny + ys[cy(j) + 1]) * nx + xs[cx(j) + 1];
-            out[j] = ldo_nt(g, j * vs4 + src * 4u);
+            out[j] = ldo(g, j * vs4 + src * 4u);
         }
         return;
     }
