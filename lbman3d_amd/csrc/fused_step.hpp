@@ -649,6 +649,39 @@ struct ShellTiles {
     int axis, cnt, idx[4], nt[3];
 };
 
+// Shell tiles of `axis` for tiles of extent T[3] (host side): the tile
+// indices along the axis that hold cells 0, 1 (open low face) or n-2, n-1
+// (open high face); cnt = 0 when both faces of the axis are periodic.
+inline ShellTiles make_shell_tiles(const BCDesc<float>& bc, int axis, const int T[3]) {
+    ShellTiles st{};
+    st.axis = axis;
+    for (int a = 0; a < 3; ++a) st.nt[a] = (bc.n[a] + T[a] - 1) / T[a];
+    const int n = bc.n[axis];
+    auto open = [&](int face) { return !(bc.qkind[face] == QB_PERIODIC && bc.ukind[face] == UB_PERIODIC); };
+    auto add = [&](int c) {
+        const int t = c / T[axis];
+        for (int i = 0; i < st.cnt; ++i)
+            if (st.idx[i] == t) return;
+        st.idx[st.cnt++] = t;
+    };
+    if (open(2 * axis))
+        for (int c = 0; c < (n < 2 ? n : 2); ++c) add(c);
+    if (open(2 * axis + 1))
+        for (int c = (n > 2 ? n - 2 : 0); c < n; ++c) add(c);
+    return st;
+}
+__host__ __device__ inline int shell_items(const ShellTiles& st) {
+    const int a1 = (st.axis + 1) % 3, a2 = (st.axis + 2) % 3;
+    return st.cnt * st.nt[a1] * st.nt[a2];
+}
+// Tile coordinates of shell item L (0 <= L < shell_items(st)).
+__host__ __device__ inline void shell_tile(const ShellTiles& st, int L, int t[3]) {
+    const int a1 = (st.axis + 1) % 3, a2 = (st.axis + 2) % 3;
+    t[st.axis] = st.idx[L % st.cnt];
+    t[a1] = (L / st.cnt) % st.nt[a1];
+    t[a2] = L / (st.cnt * st.nt[a1]);
+}
+
 // The shell cells of one axis (write mode 2 + axis) with the general kernel.
 template <int TX, int TY, int ZC>
 __global__ void __launch_bounds__(TX * TY)
@@ -658,12 +691,9 @@ fused_shell_kernel(Fields<float> f, BCDesc<float> bc, Phys<float> k, int pull, i
     __shared__ float sA[2][kNumA][G::NA];
     __shared__ float sRing[3][kNumIn][G::NR];
     const int L = blockIdx.x;
-    const int a1 = (st.axis + 1) % 3, a2 = (st.axis + 2) % 3;
-    if (L >= st.cnt * st.nt[a1] * st.nt[a2]) return;
+    if (L >= shell_items(st)) return;
     int t[3];
-    t[st.axis] = st.idx[L % st.cnt];
-    t[a1] = (L / st.cnt) % st.nt[a1];
-    t[a2] = L / (st.cnt * st.nt[a1]);
+    shell_tile(st, L, t);
     fused_body<TX, TY, ZC, false>(f, bc, k, pull, aux, t[0], t[1], t[2], sIn, sA, sRing, 2 + st.axis);
 }
 
@@ -1163,25 +1193,10 @@ template <int TX, int TY, int ZC>
 static void launch_shell(const Fields<float>& f, const BCDesc<float>& bc, const Phys<float>& k, int pull, int aux,
                          int axis, hipStream_t s) {
     const int T[3] = {TX, TY, ZC};
-    ShellTiles st{};
-    st.axis = axis;
-    for (int a = 0; a < 3; ++a) st.nt[a] = (bc.n[a] + T[a] - 1) / T[a];
-    const int n = bc.n[axis];
-    auto open = [&](int face) { return !(bc.qkind[face] == QB_PERIODIC && bc.ukind[face] == UB_PERIODIC); };
-    auto add = [&](int c) {
-        const int t = c / T[axis];
-        for (int i = 0; i < st.cnt; ++i)
-            if (st.idx[i] == t) return;
-        st.idx[st.cnt++] = t;
-    };
-    if (open(2 * axis))
-        for (int c = 0; c < std::min(2, n); ++c) add(c);
-    if (open(2 * axis + 1))
-        for (int c = std::max(n - 2, 0); c < n; ++c) add(c);
+    const ShellTiles st = make_shell_tiles(bc, axis, T);
     if (st.cnt == 0) return;
-    const int a1 = (axis + 1) % 3, a2 = (axis + 2) % 3;
-    const int blocks = st.cnt * st.nt[a1] * st.nt[a2];
-    hipLaunchKernelGGL((fused_shell_kernel<TX, TY, ZC>), dim3(blocks), dim3(TX * TY), 0, s, f, bc, k, pull, aux, st);
+    hipLaunchKernelGGL((fused_shell_kernel<TX, TY, ZC>), dim3(shell_items(st)), dim3(TX * TY), 0, s, f, bc, k, pull,
+                       aux, st);
 }
 
 static void launch_shells(const Fields<float>& f, const BCDesc<float>& bc, const Phys<float>& k, int pull, int aux,
