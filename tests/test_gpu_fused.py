@@ -2,8 +2,10 @@
 against the FP64 reference (needs a GPU).
 
 Both FP32 schedules evaluate the same pointwise formulas; they differ only in
-FMA contraction and in where intermediates live, so after a few steps they
-agree to FP32 rounding (tolerance 2e-6 relative, stated per test). Odd grid
+FMA contraction, in where intermediates live and, in the third pipeline, in
+the summation order of the body force (divergence form), so after a few
+steps they agree to FP32 rounding (tolerance 2e-6 relative, stated per
+test). Odd grid
 sizes exercise partial tiles and partial z chunks; every boundary type the
 reference has is covered.
 """
