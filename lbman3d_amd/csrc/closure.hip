@@ -84,13 +84,22 @@ conv_fwd_kernel(const __bf16* __restrict__ in, const bf16x8* __restrict__ wpk, c
     for (int t = 0; t < kTY; ++t)
 #pragma unroll
         for (int n = 0; n < NT; ++n) acc[t][n] = (f32x4){0.f, 0.f, 0.f, 0.f};
+    // B fragments (packed weights, L2-resident) are loaded one K step ahead
+    // so that their latency overlaps the MFMAs of the current step.
+    bf16x8 bnext[NT];
+#pragma unroll
+    for (int n = 0; n < NT; ++n) bnext[n] = wpk[(long)n * 64 + lane];
     for (int kc = 0; kc < KCH; ++kc) {
         const int off = CIN == 8 ? kc * 4 + kb : kc;
         const int q = CIN == 8 ? 0 : kb;
         const int dz = off / 9, dy = (off / 3) % 3, dx = off % 3;   // halo coordinates shift by +1
         bf16x8 b[NT];
 #pragma unroll
-        for (int n = 0; n < NT; ++n) b[n] = wpk[((long)kc * NT + n) * 64 + lane];
+        for (int n = 0; n < NT; ++n) b[n] = bnext[n];
+        if (kc + 1 < KCH) {
+#pragma unroll
+            for (int n = 0; n < NT; ++n) bnext[n] = wpk[((long)(kc + 1) * NT + n) * 64 + lane];
+        }
 #pragma unroll
         for (int t = 0; t < kTY; ++t) {
             bf16x8 a;
