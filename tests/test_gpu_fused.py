@@ -24,7 +24,9 @@ CASES = [("FullyPeriodic", (37, 29, 23), abi.ADV_UPWIND), ("Channel", (40, 20, 1
          ("SlitNoSlip", (80, 36, 24), abi.ADV_CENTRED), ("Channel", (72, 44, 20), abi.ADV_UPWIND),
          ("MovingLidZ", (24, 20, 48), abi.ADV_UPWIND),
          # overlapping shells: every cell within two cells of a wall
-         ("Channel", (3, 5, 10), abi.ADV_CENTRED)]
+         ("Channel", (3, 5, 10), abi.ADV_CENTRED),
+         # thin in z: one and two planes (the z march wraps onto the same planes)
+         ("FullyPeriodic", (40, 24, 2), abi.ADV_UPWIND), ("Channel", (24, 20, 1), abi.ADV_CENTRED)]
 
 
 def rel(a, b):
