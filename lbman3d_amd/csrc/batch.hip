@@ -82,13 +82,20 @@ __device__ __forceinline__ int find_sim(int nsims, int w, Get begin_of) {
     return lo;
 }
 
-// Tile shapes of the batched kernel (first fused pipeline).
+// Tile shapes of the batched kernel: kind 0 runs the first fused pipeline
+// over every tile (cells of the wall shells excluded), kind 1 the third
+// pipeline (one barrier per plane; it writes the shells too, and the shell
+// launch overwrites them, as on a single simulation with walls).
 struct BatchTile {
-    int tx, ty, zc;
+    int kind, tx, ty, zc;
     const char* name;
 };
-constexpr BatchTile kBatchTiles[] = {{32, 8, 32, "32x8x32"}, {16, 16, 32, "16x16x32"}, {32, 8, 16, "32x8x16"},
-                                     {16, 16, 16, "16x16x16"}, {16, 16, 8, "16x16x8"},   {32, 8, 8, "32x8x8"}};
+constexpr BatchTile kBatchTiles[] = {
+    {0, 32, 8, 32, "32x8x32"},       {0, 16, 16, 32, "16x16x32"},     {0, 32, 8, 16, "32x8x16"},
+    {0, 16, 16, 16, "16x16x16"},     {0, 16, 16, 8, "16x16x8"},       {0, 32, 8, 8, "32x8x8"},
+    {1, 32, 8, 64, "p2:32x8x64"},    {1, 32, 8, 32, "p2:32x8x32"},    {1, 32, 8, 16, "p2:32x8x16"},
+    {1, 64, 4, 32, "p2:64x4x32"},
+};
 constexpr int kNumBatchTiles = sizeof(kBatchTiles) / sizeof(kBatchTiles[0]);
 
 template <int TX, int TY, int ZC>
@@ -107,6 +114,28 @@ batch_step_kernel(const SimDesc* __restrict__ descs, int nsims, int total, int p
         const int a = aux | d.aux_always;
         fused_body<TX, TY, ZC, true>(d.f[parity], d.bc, d.k, d.pull, a, tx, ty, tz, sIn, sA, sRing,
                                      d.periodic ? 0 : 1);
+        __syncthreads();
+    }
+}
+
+// The same work list on the third pipeline. A simulation's advection
+// scheme picks the instantiation per work item (uniform over the block).
+template <int TX, int TY, int ZC>
+__global__ void __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(2, 2)))
+batch_step_kernel_p2(const SimDesc* __restrict__ descs, int nsims, int total, int parity, int aux) {
+    using G = TileGeom<TX, TY>;
+    __shared__ float sIn[2][kNumIn][G::NI];
+    __shared__ float sP[2][6][G::NA];
+    __shared__ float sRing[4][kNumIn][G::NR];
+    for (int w = blockIdx.x; w < total; w += gridDim.x) {
+        const SimDesc& d = descs[find_sim(nsims, w, [&](int i) { return descs[i].tile_begin; })];
+        const int local = w - d.tile_begin;
+        const int tx = local % d.ntx, ty = (local / d.ntx) % d.nty, tz = local / (d.ntx * d.nty);
+        const int a = aux | d.aux_always;
+        if (d.k.advection != 0)
+            p2_tile<TX, TY, ZC, true, true, true>(d.f[parity], d.bc, d.k, d.pull, a, tx, ty, tz, sIn, sP, sRing);
+        else
+            p2_tile<TX, TY, ZC, true, false, true>(d.f[parity], d.bc, d.k, d.pull, a, tx, ty, tz, sIn, sP, sRing);
         __syncthreads();
     }
 }
@@ -130,18 +159,47 @@ batch_shell_kernel(const SimDesc* __restrict__ descs, int nsims, int total, int 
     }
 }
 
-template <int TX, int TY, int ZC>
+template <int KIND, int TX, int TY, int ZC>
 int occupancy_blocks() {
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, batch_step_kernel<TX, TY, ZC>, TX * TY, 0) != hipSuccess)
-        return 1;
-    return std::max(1, nb);
+    hipError_t e;
+    if constexpr (KIND == 0)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, batch_step_kernel<TX, TY, ZC>, TX * TY, 0);
+    else
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, batch_step_kernel_p2<TX, TY, ZC>, TX * TY, 0);
+    return e == hipSuccess ? std::max(1, nb) : 1;
 }
 
-template <int TX, int TY, int ZC>
+template <int KIND, int TX, int TY, int ZC>
 void launch_batch(const SimDesc* d, int nsims, int total, int parity, int aux, int grid, hipStream_t s) {
-    hipLaunchKernelGGL((batch_step_kernel<TX, TY, ZC>), dim3(grid), dim3(TX * TY), 0, s, d, nsims, total, parity,
-                       aux);
+    if constexpr (KIND == 0)
+        hipLaunchKernelGGL((batch_step_kernel<TX, TY, ZC>), dim3(grid), dim3(TX * TY), 0, s, d, nsims, total,
+                           parity, aux);
+    else
+        hipLaunchKernelGGL((batch_step_kernel_p2<TX, TY, ZC>), dim3(grid), dim3(TX * TY), 0, s, d, nsims, total,
+                           parity, aux);
+}
+
+// Every batch tile shape, in catalogue order: fn(TileTag<KIND, TX, TY, ZC>{}).
+static_assert(kNumBatchTiles == 10, "the dispatch below lists every catalogue entry");
+template <int K, int X, int Y, int Z>
+struct TileTag {
+    static constexpr int kind = K, tx = X, ty = Y, zc = Z;
+};
+template <typename F>
+auto with_batch_tile(int tile, F&& fn) {
+    switch (tile) {
+        case 0: return fn(TileTag<0, 32, 8, 32>{});
+        case 1: return fn(TileTag<0, 16, 16, 32>{});
+        case 2: return fn(TileTag<0, 32, 8, 16>{});
+        case 3: return fn(TileTag<0, 16, 16, 16>{});
+        case 4: return fn(TileTag<0, 16, 16, 8>{});
+        case 5: return fn(TileTag<0, 32, 8, 8>{});
+        case 6: return fn(TileTag<1, 32, 8, 64>{});
+        case 7: return fn(TileTag<1, 32, 8, 32>{});
+        case 8: return fn(TileTag<1, 32, 8, 16>{});
+        default: return fn(TileTag<1, 64, 4, 32>{});
+    }
 }
 
 }  // namespace
@@ -186,15 +244,10 @@ int grid_for_tile(int tile, int total) {
         hipDeviceProp_t p;
         if (hipGetDeviceProperties(&p, dev) == hipSuccess) cus = p.multiProcessorCount;
     }
-    int occ = 1;
-    switch (tile) {
-        case 0: occ = occupancy_blocks<32, 8, 32>(); break;
-        case 1: occ = occupancy_blocks<16, 16, 32>(); break;
-        case 2: occ = occupancy_blocks<32, 8, 16>(); break;
-        case 3: occ = occupancy_blocks<16, 16, 16>(); break;
-        case 4: occ = occupancy_blocks<16, 16, 8>(); break;
-        default: occ = occupancy_blocks<32, 8, 8>(); break;
-    }
+    const int occ = with_batch_tile(tile, [](auto t) {
+        using T = decltype(t);
+        return occupancy_blocks<T::kind, T::tx, T::ty, T::zc>();
+    });
     return std::max(1, std::min(total, cus * occ));
 }
 
@@ -268,14 +321,10 @@ int sync_handles(lbm_batch* b) {
 
 void launch_step(lbm_batch* b, int aux) {
     const int n = (int)b->sims.size();
-    switch (b->tile) {
-        case 0: launch_batch<32, 8, 32>(b->d_desc, n, b->total, b->parity, aux, b->grid, b->stream); break;
-        case 1: launch_batch<16, 16, 32>(b->d_desc, n, b->total, b->parity, aux, b->grid, b->stream); break;
-        case 2: launch_batch<32, 8, 16>(b->d_desc, n, b->total, b->parity, aux, b->grid, b->stream); break;
-        case 3: launch_batch<16, 16, 16>(b->d_desc, n, b->total, b->parity, aux, b->grid, b->stream); break;
-        case 4: launch_batch<16, 16, 8>(b->d_desc, n, b->total, b->parity, aux, b->grid, b->stream); break;
-        default: launch_batch<32, 8, 8>(b->d_desc, n, b->total, b->parity, aux, b->grid, b->stream); break;
-    }
+    with_batch_tile(b->tile, [&](auto t) {
+        using T = decltype(t);
+        launch_batch<T::kind, T::tx, T::ty, T::zc>(b->d_desc, n, b->total, b->parity, aux, b->grid, b->stream);
+    });
     if (b->shell_total > 0)
         hipLaunchKernelGGL(batch_shell_kernel, dim3(b->shell_grid), dim3(256), 0, b->stream, b->d_desc, n,
                            b->shell_total, b->parity, aux);

@@ -959,22 +959,16 @@ __device__ __forceinline__ void p_sets(float alpha, const StageA& a, float pxy[6
 //   after it: Q update at plane pa, then force + collide-stream at plane pa - 1.
 // A wave can run at most one iteration ahead of the slowest (the next
 // barrier), which the two LDS buffers and the four ring-input slots cover.
-template <int TX, int TY, int ZC, bool MASKED, int WPE, bool UPWIND, bool NT>
-__global__ void __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(WPE, WPE)))
-fused_step_kernel_p2(Fields<float> f, BCDesc<float> bc, Phys<float> k, int pull, int aux, int ntx, int nty,
-                     int nzc) {
+// One call computes z planes [tz*ZC, tz*ZC + ZC) of the TX x TY column tile
+// (tx, ty), with LDS buffers from the calling kernel; a kernel that runs
+// several tiles per block puts a barrier between them.
+template <int TX, int TY, int ZC, bool MASKED, bool UPWIND, bool NT>
+__device__ __forceinline__ void p2_tile(const Fields<float>& f, const BCDesc<float>& bc, const Phys<float>& k,
+                                       int pull, int aux, int tx, int ty, int tz,
+                                       float (*sIn)[kNumIn][TileGeom<TX, TY>::NI],
+                                       float (*sP)[6][TileGeom<TX, TY>::NA],
+                                       float (*sRing)[kNumIn][TileGeom<TX, TY>::NR]) {
     using G = TileGeom<TX, TY>;
-    __shared__ float sIn[2][kNumIn][G::NI];        // input plane, by parity
-    __shared__ float sP[2][6][G::NA];              // X and Y sets, by plane parity
-    __shared__ float sRing[4][kNumIn][G::NR];      // ring-column inputs, planes mod 4
-
-    const int nblk = ntx * nty * nzc;
-    const int per_xcd = (nblk + 7) / 8;
-    const int L = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
-    if (L >= nblk) return;
-    // z chunks of one column are neighbours in the order (they share halo
-    // planes), then columns along x, then rows along y.
-    const int tz = L % nzc, tx = (L / nzc) % ntx, ty = L / (nzc * ntx);
     const int nx = bc.n[0], ny = bc.n[1], nz = bc.n[2];
     const int x0 = tx * TX, y0 = ty * TY;
     const int z0 = tz * ZC, z1 = min(z0 + ZC, nz);
@@ -1150,6 +1144,33 @@ fused_step_kernel_p2(Fields<float> f, BCDesc<float> bc, Phys<float> k, int pull,
     }
 }
 
+template <int TX, int TY, int ZC, bool MASKED, int WPE, bool UPWIND, bool NT, bool ZSLOW>
+__global__ void __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(WPE, WPE)))
+fused_step_kernel_p2(Fields<float> f, BCDesc<float> bc, Phys<float> k, int pull, int aux, int ntx, int nty,
+                     int nzc) {
+    using G = TileGeom<TX, TY>;
+    __shared__ float sIn[2][kNumIn][G::NI];        // input plane, by parity
+    __shared__ float sP[2][6][G::NA];              // X and Y sets, by plane parity
+    __shared__ float sRing[4][kNumIn][G::NR];      // ring-column inputs, planes mod 4
+
+    const int nblk = ntx * nty * nzc;
+    const int per_xcd = (nblk + 7) / 8;
+    const int L = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
+    if (L >= nblk) return;
+    // Two block orders. ZSLOW: columns along x, then rows along y, then z
+    // chunks, so the blocks of one XCD cover a band of whole rows and every
+    // plane they touch is one contiguous stretch of each array. Otherwise the
+    // z chunks of one column are neighbours (they share halo planes), then
+    // columns along x, then rows along y.
+    int tz, tx, ty;
+    if constexpr (ZSLOW) {
+        tx = L % ntx; ty = (L / ntx) % nty; tz = L / (ntx * nty);
+    } else {
+        tz = L % nzc; tx = (L / nzc) % ntx; ty = L / (nzc * ntx);
+    }
+    p2_tile<TX, TY, ZC, MASKED, UPWIND, NT>(f, bc, k, pull, aux, tx, ty, tz, sIn, sP, sRing);
+}
+
 #ifndef LBM_FUSED_BODY_ONLY
 // ---- tile catalogue -----------------------------------------------------------
 // kind 0: first pipeline (stage-A planes double-buffered in LDS);
@@ -1177,6 +1198,9 @@ constexpr TileShape kTiles[] = {
     // third pipeline: divergence-form sets, one barrier per plane
     {3, 64, 4, 128, "p2:64x4x128"},    {3, 32, 8, 128, "p2:32x8x128"},   {3, 64, 4, 64, "p2:64x4x64"},
     {3, 64, 4, 16, "p2:64x4x16"},      {3, 32, 8, 16, "p2:32x8x16"},
+    // third pipeline, z chunks slowest in the block order
+    {4, 64, 4, 128, "p2z:64x4x128"},   {4, 64, 4, 64, "p2z:64x4x64"},    {4, 64, 4, 32, "p2z:64x4x32"},
+    {4, 32, 8, 128, "p2z:32x8x128"},
 };
 constexpr int kNumTiles = sizeof(kTiles) / sizeof(kTiles[0]);
 
@@ -1244,8 +1268,9 @@ static void launch_tile(const Fields<float>& f, const BCDesc<float>& bc, const P
             if (up) LBM_FUSED_PF(true, true, true); else LBM_FUSED_PF(true, false, true);
         }
     } else {
-#define LBM_FUSED_P2(M, U)                                                                                    \
-    hipLaunchKernelGGL((fused_step_kernel_p2<TX, TY, ZC, M, 2, U, true>), dim3(grid), dim3(TX * TY), 0, s, f, bc, \
+        constexpr bool zs = KIND == 4;
+#define LBM_FUSED_P2(M, U)                                                                                        \
+    hipLaunchKernelGGL((fused_step_kernel_p2<TX, TY, ZC, M, 2, U, true, zs>), dim3(grid), dim3(TX * TY), 0, s, f, bc, \
                        k, pull, aux, ntx, nty, nzc)
         if (periodic) {
             if (up) LBM_FUSED_P2(false, true); else LBM_FUSED_P2(false, false);
@@ -1283,7 +1308,11 @@ void launch_fused_step_f32(const Fields<float>& f, const BCDesc<float>& bc, cons
         case 19: launch_tile<3, 32, 8, 128>(f, bc, k, pull, aux, s); break;
         case 20: launch_tile<3, 64, 4, 64>(f, bc, k, pull, aux, s); break;
         case 21: launch_tile<3, 64, 4, 16>(f, bc, k, pull, aux, s); break;
-        default: launch_tile<3, 32, 8, 16>(f, bc, k, pull, aux, s); break;
+        case 22: launch_tile<3, 32, 8, 16>(f, bc, k, pull, aux, s); break;
+        case 23: launch_tile<4, 64, 4, 128>(f, bc, k, pull, aux, s); break;
+        case 24: launch_tile<4, 64, 4, 64>(f, bc, k, pull, aux, s); break;
+        case 25: launch_tile<4, 64, 4, 32>(f, bc, k, pull, aux, s); break;
+        default: launch_tile<4, 32, 8, 128>(f, bc, k, pull, aux, s); break;
     }
 }
 
