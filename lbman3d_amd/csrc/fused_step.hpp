@@ -454,6 +454,18 @@ __device__ __forceinline__ void fused_body(const Fields<float>& f, const BCDesc<
 #pragma unroll
         for (int m = 0; m < kNumIn; ++m) sRing[(z0 - 1 + 6) % 3][m][t] = v[m];
     }
+    // Ring and outer-ring inputs are loaded one plane ahead of their LDS
+    // store: ring plane pa + 1 and outer plane pa arrive in iteration pa.
+    float rnext[kNumIn], onext[kNumIn];
+    if (has_ring) load_column<TX, TY>(f, qcr, ucr, qplane(z0), uplane(z0), rnext);
+    auto load_outer = [&](int z, float* v) {
+        const unsigned qo = (qplane(z) + (unsigned)oqc) * 4u, uo = (uplane(z) + (unsigned)ouc) * 4u;
+#pragma unroll
+        for (int m = 0; m < 5; ++m) v[m] = ldo(f.q[m], qo);
+#pragma unroll
+        for (int m = 0; m < 3; ++m) v[5 + m] = ldo(f.u[m], uo);
+    };
+    if (has_outer) load_outer(z0 - 1, onext);
 
     // z-difference values of the interior column's stage A at plane pa - 2
     // (the lower z neighbour of the plane finished in step 3), and the
@@ -471,6 +483,14 @@ __device__ __forceinline__ void fused_body(const Fields<float>& f, const BCDesc<
             hin[1][m] = hin[2][m];
         }
         load_column<TX, TY>(f, qc0, uc0, qplane(pa + 1), uplane(pa + 1), hin[2]);
+        // The pull gather of plane pb = pa - 1 reads the previous state only,
+        // so it is issued here and completes under the barrier and the Q
+        // update. (The moving-wall term reads rho of the own cell, which only
+        // this thread writes, after the collision below.)
+        const int pb = pa - 1;
+        const unsigned cpb = (unsigned)pb * plane + (unsigned)gy * nx + gx;
+        float pop[kQ];
+        if (pb >= z0 && active) gather32<PERIODIC>(f, bc, gx, gy, pb, cpb, pull, pop);
         {
             const int cc = (ib + 1) * G::IX + ia + 1;
 #pragma unroll
@@ -478,22 +498,19 @@ __device__ __forceinline__ void fused_body(const Fields<float>& f, const BCDesc<
         }
         const int r_m = (pa - 1 + 6) % 3, r_c = (pa + 6) % 3, r_p = (pa + 1 + 6) % 3;
         if (has_ring) {
-            float v[kNumIn];
-            load_column<TX, TY>(f, qcr, ucr, qplane(pa + 1), uplane(pa + 1), v);
             const int cc = (rb + 1) * G::IX + ra + 1;
 #pragma unroll
             for (int m = 0; m < kNumIn; ++m) {
-                sRing[r_p][m][t] = v[m];
+                sRing[r_p][m][t] = rnext[m];
                 sIn[m][cc] = sRing[r_c][m][t];
             }
+            if (pa < z1) load_column<TX, TY>(f, qcr, ucr, qplane(pa + 2), uplane(pa + 2), rnext);
         }
         if (has_outer) {
-            const unsigned qo = (qplane(pa) + (unsigned)oqc) * 4u, uo = (uplane(pa) + (unsigned)ouc) * 4u;
             const int cc = oj * G::IX + oi;
 #pragma unroll
-            for (int m = 0; m < 5; ++m) sIn[m][cc] = ldo(f.q[m], qo);
-#pragma unroll
-            for (int m = 0; m < 3; ++m) sIn[5 + m][cc] = ldo(f.u[m], uo);
+            for (int m = 0; m < kNumIn; ++m) sIn[m][cc] = onext[m];
+            if (pa < z1) load_outer(pa + 1, onext);
         }
         __syncthreads();
 
@@ -549,7 +566,6 @@ __device__ __forceinline__ void fused_body(const Fields<float>& f, const BCDesc<
         __syncthreads();
 
         // ---- step 3: force + collide-stream at plane pb = pa - 1 ----
-        const int pb = pa - 1;
         const float* A0 = &sA[(pb + 2) & 1][0][0];
         auto A = [&](int m, int col) { return A0[m * G::NA + col]; };
         if (pb >= z0 && active) {
@@ -596,12 +612,9 @@ __device__ __forceinline__ void fused_body(const Fields<float>& f, const BCDesc<
                 F[1] = fe_prev[1];
                 F[2] = fe_prev[2];
             }
-            const unsigned c = (unsigned)pb * plane + (unsigned)gy * nx + gx;
-            float pop[kQ];
-            gather32<PERIODIC>(f, bc, gx, gy, pb, c, pull, pop);
             float rho, u[3];
             collide_shifted(k, pop, F, rho, u);
-            const unsigned vs4 = (unsigned)f.vs * 4u, c4 = c * 4u;
+            const unsigned vs4 = (unsigned)f.vs * 4u, c4 = cpb * 4u;
             if (wmask.writes(pb)) {
 #pragma unroll
                 for (int j = 0; j < kQ; ++j) sto(f.gn, j * vs4 + c4, pop[j]);
