@@ -12,10 +12,9 @@
 // as a graph and replayed.
 //
 // Simulations with walls take the split of the single-simulation path: the
-// periodic specialisation over every tile writes the cells two or more cells
-// from every open face (write mode 1), and a second persistent launch writes
-// the wall shells from thin tiles with the full wall handling. The two
-// launches write disjoint cells and read only the previous state.
+// periodic specialisation over every tile is exact for the cells two or more
+// cells from every open face, and two one-thread-per-cell passes with the
+// full wall handling then overwrite the wall shells of every simulation.
 #define LBM_FUSED_BODY_ONLY
 #include "step_kernels.hpp"
 #include "fused_step.hpp"
@@ -39,35 +38,10 @@ struct SimDesc {
     int tile_begin;       // first work item of this simulation
     int pull;             // 0 until the first step after an upload
     int aux_always;       // moving walls read rho every step
-    int shell_begin;      // first item of this simulation in the shell launch
-    int shell_n[3];       // shell items per axis (0 for a periodic axis)
-    ShellTiles shell[3];  // shell tiles per axis
+    ShellRegion r3, r2;   // shell and halo (Q pass), shell alone (force pass); empty if periodic
+    int sq_begin;         // first item of this simulation in the shell Q pass
+    int sf_begin;         // first item of this simulation in the shell force pass
 };
-
-// Shell tile shapes (256 threads like the interior tiles): 8 cells across an
-// x or y wall (32-cell rows fit the small boxes of a sweep), 2 planes across a
-// z wall. Over the 64-simulation sweep, 8x32x8 measured 0.754 ms per step
-// against 0.770-0.913 ms for 4x64 tiles of 4-16 planes and 8x32x4.
-constexpr int kShellT[3][3] = {{8, 32, 8}, {32, 8, 8}, {16, 16, 2}};
-
-template <int TX, int TY>
-struct LdsSize {
-    using G = TileGeom<TX, TY>;
-    static constexpr int value = kNumIn * G::NI + 2 * kNumA * G::NA + 3 * kNumIn * G::NR;
-};
-constexpr int cmax(int a, int b) { return a > b ? a : b; }
-constexpr int kShellLds = cmax(cmax(LdsSize<8, 32>::value, LdsSize<32, 8>::value), LdsSize<16, 16>::value);
-
-// fused_body with its LDS carved from one shared buffer.
-template <int TX, int TY, int ZC, bool PERIODIC>
-__device__ __forceinline__ void body_in(float* smem, const Fields<float>& f, const BCDesc<float>& bc,
-                                        const Phys<float>& k, int pull, int aux, const int t[3], int mode) {
-    using G = TileGeom<TX, TY>;
-    auto sIn = reinterpret_cast<float (*)[G::NI]>(smem);
-    auto sA = reinterpret_cast<float (*)[kNumA][G::NA]>(smem + kNumIn * G::NI);
-    auto sRing = reinterpret_cast<float (*)[kNumIn][G::NR]>(smem + kNumIn * G::NI + 2 * kNumA * G::NA);
-    fused_body<TX, TY, ZC, PERIODIC>(f, bc, k, pull, aux, t[0], t[1], t[2], sIn, sA, sRing, mode);
-}
 
 // Last simulation whose range starts at or before item w (ranges ascending;
 // empty ranges share their start with the next simulation).
@@ -140,23 +114,30 @@ batch_step_kernel_p2(const SimDesc* __restrict__ descs, int nsims, int total, in
     }
 }
 
-// Wall shells of every simulation with walls (write modes 2 + axis).
-__global__ void __launch_bounds__(256)
-batch_shell_kernel(const SimDesc* __restrict__ descs, int nsims, int total, int parity, int aux) {
-    __shared__ __align__(16) float smem[kShellLds];
-    for (int w = blockIdx.x; w < total; w += gridDim.x) {
-        const SimDesc& d = descs[find_sim(nsims, w, [&](int i) { return descs[i].shell_begin; })];
-        int local = w - d.shell_begin;
-        int axis = 0;
-        while (axis < 2 && local >= d.shell_n[axis]) local -= d.shell_n[axis++];
-        int t[3];
-        shell_tile(d.shell[axis], local, t);
-        const int a = aux | d.aux_always;
-        if (axis == 0) body_in<8, 32, 8, false>(smem, d.f[parity], d.bc, d.k, d.pull, a, t, 2);
-        else if (axis == 1) body_in<32, 8, 8, false>(smem, d.f[parity], d.bc, d.k, d.pull, a, t, 3);
-        else body_in<16, 16, 2, false>(smem, d.f[parity], d.bc, d.k, d.pull, a, t, 4);
-        __syncthreads();
-    }
+// Wall shells of every simulation with walls, one thread per cell, in the
+// two passes of the single-simulation path: the Q update over each shell and
+// its one-cell halo, then body force and collide-stream of the shell cells.
+__global__ void __launch_bounds__(kBlock)
+batch_shell_q_kernel(const SimDesc* __restrict__ descs, int nsims, int total, int parity) {
+    const int w = blockIdx.x * kBlock + threadIdx.x;
+    if (w >= total) return;
+    const SimDesc& d = descs[find_sim(nsims, w, [&](int i) { return descs[i].sq_begin; })];
+    const Pos p = region_cell(d.r3, d.bc.n, w - d.sq_begin);
+    const long c = cell_of(p.x, p.y, p.z, d.bc.n[0], d.bc.n[1]);
+    qstep_cell(d.f[parity], d.bc, d.k, c, p, !cell_mask(d.bc, 1, p.x, p.y).writes(p.z));
+}
+
+__global__ void __launch_bounds__(kBlock)
+batch_shell_force_kernel(const SimDesc* __restrict__ descs, int nsims, int total, int parity, int aux) {
+    const int w = blockIdx.x * kBlock + threadIdx.x;
+    if (w >= total) return;
+    const SimDesc& d = descs[find_sim(nsims, w, [&](int i) { return descs[i].sf_begin; })];
+    const Pos p = region_cell(d.r2, d.bc.n, w - d.sf_begin);
+    Fields<float> fq = d.f[parity];
+    for (int m = 0; m < 5; ++m) fq.q[m] = fq.qn[m];   // the force reads Q^{n+1}
+    const int flags = FL_FORCE | FL_LBM | ((aux | d.aux_always) ? (FL_STORE_F | FL_STORE_RHO) : 0);
+    force_lbm_cell<float, true, true>(fq, d.bc, d.k, cell_of(p.x, p.y, p.z, d.bc.n[0], d.bc.n[1]), p, flags,
+                                      d.pull);
 }
 
 template <int KIND, int TX, int TY, int ZC>
@@ -210,11 +191,11 @@ struct lbm_batch {
     SimDesc* d_desc = nullptr;
     hipStream_t stream = nullptr;
     int total = 0;
-    int shell_total = 0;   // work items of the shell launch (simulations with walls)
+    int sq_total = 0;      // cells of the shell Q pass (simulations with walls)
+    int sf_total = 0;      // cells of the shell force pass
     int parity = 0;        // how far the buffers have ping-ponged since the handles were synced
     int tile = 0;
     int grid = 0;
-    int shell_grid = 0;
     bool desc_dirty = true;
     hipGraphExec_t graph[2] = {nullptr, nullptr};   // two steps starting at parity 0 / 1
 };
@@ -251,21 +232,10 @@ int grid_for_tile(int tile, int total) {
     return std::max(1, std::min(total, cus * occ));
 }
 
-int grid_for_shells(int total) {
-    if (total <= 0) return 0;
-    int dev = 0, cus = 256, occ = 1;
-    if (hipGetDevice(&dev) == hipSuccess) {
-        hipDeviceProp_t p;
-        if (hipGetDeviceProperties(&p, dev) == hipSuccess) cus = p.multiProcessorCount;
-    }
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, batch_shell_kernel, 256, 0) != hipSuccess) occ = 1;
-    return std::max(1, std::min(total, cus * std::max(1, occ)));
-}
-
 // Build the descriptors from the handles' current buffers (parity 0 view).
 int rebuild(lbm_batch* b) {
     b->desc.resize(b->sims.size());
-    int begin = 0, shell_begin = 0;
+    int begin = 0, sq_begin = 0, sf_begin = 0;
     for (size_t i = 0; i < b->sims.size(); ++i) {
         lbm_solver* s = b->sims[i];
         SimDesc& d = b->desc[i];
@@ -283,17 +253,17 @@ int rebuild(lbm_batch* b) {
         begin += tiles_of(b, (int)i, d.ntx, d.nty, d.nzc);
         d.pull = s->pulled;
         d.aux_always = s->moving ? 1 : 0;
-        d.shell_begin = shell_begin;
-        for (int a = 0; a < 3; ++a) {
-            d.shell[a] = make_shell_tiles(d.bc, a, kShellT[a]);
-            d.shell_n[a] = d.periodic ? 0 : shell_items(d.shell[a]);
-            shell_begin += d.shell_n[a];
-        }
+        d.r3 = make_shell_region(d.bc, 3);
+        d.r2 = make_shell_region(d.bc, 2);
+        d.sq_begin = sq_begin;
+        d.sf_begin = sf_begin;
+        sq_begin += (int)region_cells(d.r3);
+        sf_begin += (int)region_cells(d.r2);
     }
     b->total = begin;
     b->grid = grid_for_tile(b->tile, b->total);
-    b->shell_total = shell_begin;
-    b->shell_grid = grid_for_shells(b->shell_total);
+    b->sq_total = sq_begin;
+    b->sf_total = sf_begin;
     if (hipMemcpyAsync(b->d_desc, b->desc.data(), b->desc.size() * sizeof(SimDesc), hipMemcpyHostToDevice,
                        b->stream) != hipSuccess)
         return fail(LBM_ERR_HIP, "descriptor upload failed");
@@ -325,9 +295,12 @@ void launch_step(lbm_batch* b, int aux) {
         using T = decltype(t);
         launch_batch<T::kind, T::tx, T::ty, T::zc>(b->d_desc, n, b->total, b->parity, aux, b->grid, b->stream);
     });
-    if (b->shell_total > 0)
-        hipLaunchKernelGGL(batch_shell_kernel, dim3(b->shell_grid), dim3(256), 0, b->stream, b->d_desc, n,
-                           b->shell_total, b->parity, aux);
+    if (b->sf_total > 0) {
+        hipLaunchKernelGGL(batch_shell_q_kernel, dim3(grid_for(b->sq_total)), dim3(kBlock), 0, b->stream, b->d_desc,
+                           n, b->sq_total, b->parity);
+        hipLaunchKernelGGL(batch_shell_force_kernel, dim3(grid_for(b->sf_total)), dim3(kBlock), 0, b->stream,
+                           b->d_desc, n, b->sf_total, b->parity, aux);
+    }
 }
 
 // One step outside any graph; afterwards every simulation pulls.

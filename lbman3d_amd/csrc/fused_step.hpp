@@ -21,6 +21,8 @@
 // the reference clamps.
 #pragma once
 
+#include <algorithm>
+
 #include "launch.hpp"
 #include "physics.hpp"
 #include "step_kernels.hpp"
@@ -351,7 +353,8 @@ __device__ __forceinline__ float stage_value(const StageA& s, int m) {
 // 1 + axis.
 //
 // Write modes of a launch: 0 every cell, 1 class 0 only, 2 + a the shell of
-// axis a. A mode gates the stores only; the force stage runs for every
+// axis a (the launches use modes 0 and 1; the shells are written by the
+// per-cell shell passes). A mode gates the stores only; the force stage runs for every
 // active cell as in an unmasked launch. (With the whole force stage under the
 // mask, the build for gfx950 applied the Neumann ghost across a periodic z
 // face in the lanes that wrote, at the first and last planes; gating the
@@ -656,59 +659,87 @@ fused_step_kernel(Fields<float> f, BCDesc<float> bc, Phys<float> k, int pull, in
     fused_body<TX, TY, ZC, PERIODIC>(f, bc, k, pull, aux, tx, ty, tz, sIn, sA, sRing, mode);
 }
 
-// Tiles of one wall shell: along `axis`, the tile indices idx[0..cnt) that
-// hold shell cells; along the other two axes, every tile.
-struct ShellTiles {
-    int axis, cnt, idx[4], nt[3];
+// ---- wall shells of a single simulation, one thread per cell ------------------
+// The cells within w of an open face, each once and in the class order of
+// the shells: the x band, then the y band outside the x band, then the z band
+// outside both. Along axis a the band is [0, lo) and [hi, n); its complement
+// [lo, hi) is one interval.
+struct ShellRegion {
+    int lo[3], hi[3];
+    long part[3];   // cells of the x, y and z parts
 };
 
-// Shell tiles of `axis` for tiles of extent T[3] (host side): the tile
-// indices along the axis that hold cells 0, 1 (open low face) or n-2, n-1
-// (open high face); cnt = 0 when both faces of the axis are periodic.
-inline ShellTiles make_shell_tiles(const BCDesc<float>& bc, int axis, const int T[3]) {
-    ShellTiles st{};
-    st.axis = axis;
-    for (int a = 0; a < 3; ++a) st.nt[a] = (bc.n[a] + T[a] - 1) / T[a];
-    const int n = bc.n[axis];
-    auto open = [&](int face) { return !(bc.qkind[face] == QB_PERIODIC && bc.ukind[face] == UB_PERIODIC); };
-    auto add = [&](int c) {
-        const int t = c / T[axis];
-        for (int i = 0; i < st.cnt; ++i)
-            if (st.idx[i] == t) return;
-        st.idx[st.cnt++] = t;
-    };
-    if (open(2 * axis))
-        for (int c = 0; c < (n < 2 ? n : 2); ++c) add(c);
-    if (open(2 * axis + 1))
-        for (int c = (n > 2 ? n - 2 : 0); c < n; ++c) add(c);
-    return st;
+inline ShellRegion make_shell_region(const BCDesc<float>& bc, int w) {
+    ShellRegion r{};
+    int band[3];
+    for (int a = 0; a < 3; ++a) {
+        const int n = bc.n[a];
+        const bool open_lo = !(bc.qkind[2 * a] == QB_PERIODIC && bc.ukind[2 * a] == UB_PERIODIC);
+        const bool open_hi = !(bc.qkind[2 * a + 1] == QB_PERIODIC && bc.ukind[2 * a + 1] == UB_PERIODIC);
+        r.lo[a] = open_lo ? std::min(w, n) : 0;
+        r.hi[a] = open_hi ? std::max(r.lo[a], n - w) : n;
+        band[a] = r.lo[a] + (n - r.hi[a]);
+    }
+    const long cx = bc.n[0] - band[0], cy = bc.n[1] - band[1];
+    r.part[0] = (long)band[0] * bc.n[1] * bc.n[2];
+    r.part[1] = cx * band[1] * bc.n[2];
+    r.part[2] = cx * cy * band[2];
+    return r;
 }
-__host__ __device__ inline int shell_items(const ShellTiles& st) {
-    const int a1 = (st.axis + 1) % 3, a2 = (st.axis + 2) % 3;
-    return st.cnt * st.nt[a1] * st.nt[a2];
-}
-// Tile coordinates of shell item L (0 <= L < shell_items(st)).
-__host__ __device__ inline void shell_tile(const ShellTiles& st, int L, int t[3]) {
-    const int a1 = (st.axis + 1) % 3, a2 = (st.axis + 2) % 3;
-    t[st.axis] = st.idx[L % st.cnt];
-    t[a1] = (L / st.cnt) % st.nt[a1];
-    t[a2] = L / (st.cnt * st.nt[a1]);
+__host__ __device__ inline long region_cells(const ShellRegion& r) { return r.part[0] + r.part[1] + r.part[2]; }
+
+// Cell of item i (0 <= i < region_cells(r)); x runs fastest in every part.
+__device__ __forceinline__ Pos region_cell(const ShellRegion& r, const int n[3], long i) {
+    auto pick = [&](int a, int j) { return j < r.lo[a] ? j : r.hi[a] + (j - r.lo[a]); };
+    const int bx = r.lo[0] + (n[0] - r.hi[0]), by = r.lo[1] + (n[1] - r.hi[1]);
+    const int cx = r.hi[0] - r.lo[0], cy = r.hi[1] - r.lo[1];
+    Pos p;
+    if (i < r.part[0]) {
+        const long rest = i / bx;
+        p.x = pick(0, (int)(i - rest * bx));
+        p.y = (int)(rest % n[1]);
+        p.z = (int)(rest / n[1]);
+        return p;
+    }
+    i -= r.part[0];
+    if (i < r.part[1]) {
+        const long rest = i / cx;
+        p.x = r.lo[0] + (int)(i - rest * cx);
+        p.y = pick(1, (int)(rest % by));
+        p.z = (int)(rest / by);
+        return p;
+    }
+    i -= r.part[1];
+    const long rest = i / cx;
+    p.x = r.lo[0] + (int)(i - rest * cx);
+    p.y = r.lo[1] + (int)(rest % cy);
+    p.z = pick(2, (int)(rest / cy));
+    return p;
 }
 
-// The shell cells of one axis (write mode 2 + axis) with the general kernel.
-template <int TX, int TY, int ZC>
-__global__ void __launch_bounds__(TX * TY)
-fused_shell_kernel(Fields<float> f, BCDesc<float> bc, Phys<float> k, int pull, int aux, ShellTiles st) {
-    using G = TileGeom<TX, TY>;
-    __shared__ float sIn[kNumIn][G::NI];
-    __shared__ float sA[2][kNumA][G::NA];
-    __shared__ float sRing[3][kNumIn][G::NR];
-    const int L = blockIdx.x;
-    if (L >= shell_items(st)) return;
-    int t[3];
-    shell_tile(st, L, t);
-    fused_body<TX, TY, ZC, false>(f, bc, k, pull, aux, t[0], t[1], t[2], sIn, sA, sRing, 2 + st.axis);
+#ifndef LBM_FUSED_BODY_ONLY
+// Q update over the shell and its one-cell halo (w = 3). Every cell stores
+// Sigma and Tau for the force pass; only shell cells store Q^{n+1} and the
+// Ericksen force (the interior launch owns the rest).
+static __global__ void __launch_bounds__(kBlock)
+shell_qstep_kernel(Fields<float> f, BCDesc<float> bc, Phys<float> k, ShellRegion r) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= region_cells(r)) return;
+    const Pos p = region_cell(r, bc.n, i);
+    const long c = cell_of(p.x, p.y, p.z, bc.n[0], bc.n[1]);
+    qstep_cell(f, bc, k, c, p, !cell_mask(bc, 1, p.x, p.y).writes(p.z));
 }
+
+// Body force and collide-stream of the shell cells (w = 2); f.q points at
+// Q^{n+1}, and the velocity goes to un.
+static __global__ void __launch_bounds__(kBlock)
+shell_force_lbm_kernel(Fields<float> f, BCDesc<float> bc, Phys<float> k, ShellRegion r, int flags, int pull) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= region_cells(r)) return;
+    const Pos p = region_cell(r, bc.n, i);
+    force_lbm_cell<float, true, true>(f, bc, k, cell_of(p.x, p.y, p.z, bc.n[0], bc.n[1]), p, flags, pull);
+}
+#endif  // LBM_FUSED_BODY_ONLY
 
 // Second pipeline: the same computation with one stage-A plane in LDS (the
 // force stage reads plane pb while the fresh plane pa waits in registers and
@@ -1220,27 +1251,23 @@ constexpr int kNumTiles = sizeof(kTiles) / sizeof(kTiles[0]);
 int fused_tile_count() { return kNumTiles; }
 const char* fused_tile_name(int tile) { return (tile >= 0 && tile < kNumTiles) ? kTiles[tile].name : "?"; }
 
-// Shell launches of a domain with walls. Shell tiles are 4 cells across the
-// wall-normal axis (x, y) or 2 planes (z), so little of the work they
-// recompute lies outside the shell: 4x64 / 64x4 column tiles marching 4
-// planes for the x and y shells, 32x8 tiles of 2 planes for z. (At 256^3 the
-// x shell costs 0.27-0.40 ms over the tile shapes 4x16..4x64 and chunks of
-// 4..16 planes; 4x64x4 was the fastest.)
-template <int TX, int TY, int ZC>
-static void launch_shell(const Fields<float>& f, const BCDesc<float>& bc, const Phys<float>& k, int pull, int aux,
-                         int axis, hipStream_t s) {
-    const int T[3] = {TX, TY, ZC};
-    const ShellTiles st = make_shell_tiles(bc, axis, T);
-    if (st.cnt == 0) return;
-    hipLaunchKernelGGL((fused_shell_kernel<TX, TY, ZC>), dim3(shell_items(st)), dim3(TX * TY), 0, s, f, bc, k, pull,
-                       aux, st);
-}
-
+// Shell launches of a domain with walls: after the interior launch, the
+// unfused step kernels' per-cell code recomputes the cells within two cells
+// of an open face, with full wall handling, in two passes over the shell
+// (one thread per cell, so the few shell cells of a large or small grid are
+// spread over the whole GPU). Both passes read the previous state, the new
+// Q^{n+1} of the interior launch next to the shell, and Sigma/Tau that the
+// first pass writes.
 static void launch_shells(const Fields<float>& f, const BCDesc<float>& bc, const Phys<float>& k, int pull, int aux,
                           hipStream_t s) {
-    launch_shell<4, 64, 4>(f, bc, k, pull, aux, 0, s);
-    launch_shell<64, 4, 4>(f, bc, k, pull, aux, 1, s);
-    launch_shell<32, 8, 2>(f, bc, k, pull, aux, 2, s);
+    const ShellRegion r3 = make_shell_region(bc, 3), r2 = make_shell_region(bc, 2);
+    if (region_cells(r2) == 0) return;
+    hipLaunchKernelGGL(shell_qstep_kernel, dim3(grid_for(region_cells(r3))), dim3(kBlock), 0, s, f, bc, k, r3);
+    Fields<float> fq = f;
+    for (int m = 0; m < 5; ++m) fq.q[m] = f.qn[m];
+    const int flags = FL_FORCE | FL_LBM | (aux ? (FL_STORE_F | FL_STORE_RHO) : 0);
+    hipLaunchKernelGGL(shell_force_lbm_kernel, dim3(grid_for(region_cells(r2))), dim3(kBlock), 0, s, fq, bc, k, r2,
+                       flags, pull);
 }
 
 template <int KIND, int TX, int TY, int ZC>
@@ -1259,11 +1286,9 @@ static void launch_tile(const Fields<float>& f, const BCDesc<float>& bc, const P
     // Periodic domains run one launch. On domains with walls the periodic
     // specialisation of the chosen pipeline runs over every tile; its values
     // are exact for the cells two or more cells away from every open face.
-    // The first pipeline with full wall handling then writes the shell of
-    // each axis with an open face, from tiles that are thin across that
-    // axis, over the prefetch pipeline's shell values. Each launch reads the
-    // previous state only; rho, which the shells read for moving walls, is
-    // written by the owner of each cell alone.
+    // The two shell passes then overwrite the cells within two cells of an
+    // open face. rho, which the shell read for moving walls, is written by
+    // the owner of each cell alone.
     const int mode = periodic ? 0 : 1;
     if constexpr (KIND == 0) {
         hipLaunchKernelGGL((fused_step_kernel<TX, TY, ZC, true>), dim3(grid), dim3(TX * TY), 0, s, f, bc, k, pull,

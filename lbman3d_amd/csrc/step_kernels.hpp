@@ -77,12 +77,12 @@ __device__ __forceinline__ void gather(const Fields<T>& f, const BCDesc<T>& bc, 
 
 // ---- Q update ----------------------------------------------------------------
 // QTensorSolver::StepAndSetupBodyForce (src/qtensor_solver.tpp:50-147).
+// One cell. `own` false stores Sigma and Tau only (a halo cell of a region
+// pass whose Q^{n+1} and Ericksen force belong to another launch).
 template <typename T>
-__global__ void __launch_bounds__(kBlock) qstep_kernel(Fields<T> f, BCDesc<T> bc, Phys<T> k, long V) {
-    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= V) return;
+__device__ __forceinline__ void qstep_cell(const Fields<T>& f, const BCDesc<T>& bc, const Phys<T>& k, long c, Pos p,
+                                           bool own) {
     const int nx = bc.n[0], ny = bc.n[1], nz = bc.n[2];
-    const Pos p = cell_pos(c, nx, ny);
     const int pos[3] = {p.x, p.y, p.z};
 
     // Q stencil neighbours (Q tags), velocity neighbours (velocity tags).
@@ -124,15 +124,22 @@ __global__ void __launch_bounds__(kBlock) qstep_kernel(Fields<T> f, BCDesc<T> bc
     T qn[5], sg[5], ta[3], fe[3];
     q_pointwise(k, Q, u, dq, g, qn, sg, ta, fe);
 #pragma unroll
-    for (int m = 0; m < 5; ++m) {
-        f.qn[m][c] = qn[m];
-        f.sig[m][c] = sg[m];
-    }
+    for (int m = 0; m < 5; ++m) f.sig[m][c] = sg[m];
 #pragma unroll
-    for (int m = 0; m < 3; ++m) {
-        f.tau[m][c] = ta[m];
-        f.F[m][c] = fe[m];
+    for (int m = 0; m < 3; ++m) f.tau[m][c] = ta[m];
+    if (own) {
+#pragma unroll
+        for (int m = 0; m < 5; ++m) f.qn[m][c] = qn[m];
+#pragma unroll
+        for (int m = 0; m < 3; ++m) f.F[m][c] = fe[m];
     }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) qstep_kernel(Fields<T> f, BCDesc<T> bc, Phys<T> k, long V) {
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= V) return;
+    qstep_cell(f, bc, k, c, cell_pos(c, bc.n[0], bc.n[1]), true);
 }
 
 // ---- body force and collide-stream ----------------------------------------
@@ -140,13 +147,12 @@ __global__ void __launch_bounds__(kBlock) qstep_kernel(Fields<T> f, BCDesc<T> bc
 // LatticeBoltzmannStep (src/lbm_solver.tpp:89-224) at one node. Fusing the two
 // is the merge the reference documents as valid (src/qtensor_solver.tpp:219-247):
 // the force at a node feeds only that node's collision.
-template <typename T, bool SHIFTED>
-__global__ void __launch_bounds__(kBlock)
-force_lbm_kernel(Fields<T> f, BCDesc<T> bc, Phys<T> k, long V, int flags, int pull) {
-    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= V) return;
+// One cell. UN: the velocity goes to un (fused-step buffers, where u still
+// holds u^{n-1} for other cells) instead of replacing u in place.
+template <typename T, bool SHIFTED, bool UN>
+__device__ __forceinline__ void force_lbm_cell(const Fields<T>& f, const BCDesc<T>& bc, const Phys<T>& k, long c,
+                                               Pos p, int flags, int pull) {
     const int nx = bc.n[0], ny = bc.n[1], nz = bc.n[2];
-    const Pos p = cell_pos(c, nx, ny);
     T F[3];
     if (flags & FL_FORCE) {
         const T fe[3] = {f.F[0][c], f.F[1][c], f.F[2][c]};
@@ -202,11 +208,20 @@ force_lbm_kernel(Fields<T> f, BCDesc<T> bc, Phys<T> k, long V, int flags, int pu
     if constexpr (SHIFTED) collide_shifted(k, pop, F, rho, u);
     else collide_plain(k, pop, F, rho, u);
     if (flags & FL_STORE_RHO) f.rho[c] = rho;
-    f.u[0][c] = u[0];
-    f.u[1][c] = u[1];
-    f.u[2][c] = u[2];
+    T* const* uo = UN ? f.un : f.u;
+    uo[0][c] = u[0];
+    uo[1][c] = u[1];
+    uo[2][c] = u[2];
 #pragma unroll
     for (int j = 0; j < kQ; ++j) f.gn[j * f.vs + c] = pop[j];
+}
+
+template <typename T, bool SHIFTED>
+__global__ void __launch_bounds__(kBlock)
+force_lbm_kernel(Fields<T> f, BCDesc<T> bc, Phys<T> k, long V, int flags, int pull) {
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= V) return;
+    force_lbm_cell<T, SHIFTED, false>(f, bc, k, c, cell_pos(c, bc.n[0], bc.n[1]), flags, pull);
 }
 
 template <typename T>
