@@ -96,12 +96,22 @@ batch_step_kernel(const SimDesc* __restrict__ descs, int nsims, int total, int p
 // scheme picks the instantiation per work item (uniform over the block).
 template <int TX, int TY, int ZC>
 __global__ void __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(2, 2)))
-batch_step_kernel_p2(const SimDesc* __restrict__ descs, int nsims, int total, int parity, int aux) {
+batch_step_kernel_p2(const SimDesc* __restrict__ descs, int nsims, int total, int parity, int aux,
+                     unsigned* __restrict__ queue) {
     using G = TileGeom<TX, TY>;
     __shared__ float sIn[2][kNumIn][G::NI];
     __shared__ float sP[2][6][G::NA];
     __shared__ float sRing[4][kNumIn][G::NR];
-    for (int w = blockIdx.x; w < total; w += gridDim.x) {
+    __shared__ int s_item;
+    // Work queue: blocks take the next item as they finish one, which
+    // balances tiles of uneven length. queue[0] (next item) only grows, so
+    // every block leaves once it passes total; the last block to leave
+    // (queue[1] counts them) resets both for the next launch on the stream.
+    for (;;) {
+        if (threadIdx.x == 0) s_item = (int)atomicAdd(&queue[0], 1u);
+        __syncthreads();
+        const int w = s_item;
+        if (w >= total) break;
         const SimDesc& d = descs[find_sim(nsims, w, [&](int i) { return descs[i].tile_begin; })];
         const int local = w - d.tile_begin;
         const int tx = local % d.ntx, ty = (local / d.ntx) % d.nty, tz = local / (d.ntx * d.nty);
@@ -111,6 +121,13 @@ batch_step_kernel_p2(const SimDesc* __restrict__ descs, int nsims, int total, in
         else
             p2_tile<TX, TY, ZC, true, false, true>(d.f[parity], d.bc, d.k, d.pull, a, tx, ty, tz, sIn, sP, sRing);
         __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        __threadfence();
+        if (atomicAdd(&queue[1], 1u) == gridDim.x - 1) {
+            atomicExch(&queue[0], 0u);
+            atomicExch(&queue[1], 0u);
+        }
     }
 }
 
@@ -152,13 +169,14 @@ int occupancy_blocks() {
 }
 
 template <int KIND, int TX, int TY, int ZC>
-void launch_batch(const SimDesc* d, int nsims, int total, int parity, int aux, int grid, hipStream_t s) {
+void launch_batch(const SimDesc* d, int nsims, int total, int parity, int aux, int grid, unsigned* queue,
+                  hipStream_t s) {
     if constexpr (KIND == 0)
         hipLaunchKernelGGL((batch_step_kernel<TX, TY, ZC>), dim3(grid), dim3(TX * TY), 0, s, d, nsims, total,
                            parity, aux);
     else
         hipLaunchKernelGGL((batch_step_kernel_p2<TX, TY, ZC>), dim3(grid), dim3(TX * TY), 0, s, d, nsims, total,
-                           parity, aux);
+                           parity, aux, queue);
 }
 
 // Every batch tile shape, in catalogue order: fn(TileTag<KIND, TX, TY, ZC>{}).
@@ -189,6 +207,7 @@ struct lbm_batch {
     std::vector<lbm_handle> sims;
     std::vector<SimDesc> desc;
     SimDesc* d_desc = nullptr;
+    unsigned* d_queue = nullptr;   // work-queue counters of the third-pipeline kernel (reset by the kernel)
     hipStream_t stream = nullptr;
     int total = 0;
     int sq_total = 0;      // cells of the shell Q pass (simulations with walls)
@@ -293,7 +312,8 @@ void launch_step(lbm_batch* b, int aux) {
     const int n = (int)b->sims.size();
     with_batch_tile(b->tile, [&](auto t) {
         using T = decltype(t);
-        launch_batch<T::kind, T::tx, T::ty, T::zc>(b->d_desc, n, b->total, b->parity, aux, b->grid, b->stream);
+        launch_batch<T::kind, T::tx, T::ty, T::zc>(b->d_desc, n, b->total, b->parity, aux, b->grid, b->d_queue,
+                                                   b->stream);
     });
     if (b->sf_total > 0) {
         hipLaunchKernelGGL(batch_shell_q_kernel, dim3(grid_for(b->sq_total)), dim3(kBlock), 0, b->stream, b->d_desc,
@@ -394,6 +414,9 @@ int lbm_batch_create(const lbm_config* cfgs, int32_t count, lbm_batch_handle* ou
         return bail(fail(LBM_ERR_HIP, "hipStreamCreate failed"));
     if (hipMalloc(&b->d_desc, count * sizeof(SimDesc)) != hipSuccess)
         return bail(fail(LBM_ERR_HIP, "hipMalloc(descriptors) failed"));
+    if (hipMalloc(&b->d_queue, 2 * sizeof(unsigned)) != hipSuccess ||
+        hipMemsetAsync(b->d_queue, 0, 2 * sizeof(unsigned), b->stream) != hipSuccess)
+        return bail(fail(LBM_ERR_HIP, "hipMalloc(work queue) failed"));
     if (int r = rebuild(b)) return bail(r);
     *out = b;
     return LBM_OK;
@@ -405,6 +428,7 @@ int lbm_batch_destroy(lbm_batch_handle b) {
     drop_graphs(b);
     for (lbm_handle h : b->sims) lbm_destroy(h);
     if (b->d_desc) (void)hipFree(b->d_desc);
+    if (b->d_queue) (void)hipFree(b->d_queue);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
     return LBM_OK;
