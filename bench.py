@@ -76,7 +76,10 @@ def time_solver(cfg, q, steps, warmup, schedule=None, autotune=False):
     return s, ev_ms, wall, prof, diag, tuned
 
 
-def roofline(prof, V):
+def roofline(prof, V, schedule_name=None):
+    """Roofline of the dominant kernel. traffic is the PMC-measured HBM bytes
+    per launch of the schedule that ran, from the committed profile (null
+    when that schedule has no measurement)."""
     name = max(prof, key=lambda k: prof[k])
     ms = prof[name]
     algo = ALGO_BYTES.get(name)
@@ -87,7 +90,9 @@ def roofline(prof, V):
     pmc = os.path.join(PROFILE_DIR, f"{name}_256cube_traffic.json")
     if os.path.exists(pmc):
         with open(pmc) as fp:
-            traffic = json.load(fp).get("hbm_bytes_per_launch")
+            entry = json.load(fp).get("by_schedule", {}).get(schedule_name)
+        if entry:
+            traffic = entry["hbm_bytes_per_launch"]
     return name, {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                   "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": name,
                   "kernel_ms": round(ms, 4), "algorithmic_bytes_per_launch": algo * V}
@@ -156,7 +161,7 @@ def main():
     s.close()
     ms_step = ev_ms / args.steps
     mlups = V / (ms_step * 1e-3) / 1e6
-    kname, roof = roofline(prof, V)
+    kname, roof = roofline(prof, V, schedule_used["name"])
     step_frac = STEP_BYTES["fp32"] * V / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS
     log(f"FP32: {ms_step:.4f} ms/step, {mlups:.1f} MLUPS, kernels {prof}")
 
