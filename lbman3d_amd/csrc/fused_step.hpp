@@ -354,8 +354,8 @@ __device__ __forceinline__ float stage_value(const StageA& s, int m) {
 //
 // Write modes of a launch: 0 every cell, 1 class 0 only, 2 + a the shell of
 // axis a (the launches use modes 0 and 1; the shells are written by the
-// per-cell shell passes). A mode gates the stores only; the force stage runs for every
-// active cell as in an unmasked launch. (With the whole force stage under the
+// per-cell shell passes). A mode gates the stores only; the force stage runs
+// for every active cell as in an unmasked launch. (With the whole force stage under the
 // mask, the build for gfx950 applied the Neumann ghost across a periodic z
 // face in the lanes that wrote, at the first and last planes; gating the
 // stores alone gives the reference values.)
@@ -747,9 +747,9 @@ shell_force_lbm_kernel(Fields<float> f, BCDesc<float> bc, Phys<float> k, ShellRe
 // plane ahead of its use: the pull gather of plane pb and the input planes
 // pa + 2 are in flight while the Q update of plane pa computes. Only the
 // periodic specialisation is built. On a domain with walls (MASKED) its
-// values for the shell cells are wrong and the shell launches that follow
-// overwrite them; rho and F, of which rho is also read by the shells, are
-// written for class-0 cells only.
+// values for the shell cells are wrong and the shell passes that follow
+// overwrite them; rho and F, of which rho is also read by the shell passes,
+// are written for class-0 cells only.
 template <int TX, int TY, int ZC, bool MASKED, int WPE, bool UPWIND, bool NT>
 __global__ void __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(WPE, WPE)))
 fused_step_kernel_pf(Fields<float> f, BCDesc<float> bc, Phys<float> k, int pull, int aux, int ntx, int nty,
@@ -1287,8 +1287,8 @@ static void launch_tile(const Fields<float>& f, const BCDesc<float>& bc, const P
     // specialisation of the chosen pipeline runs over every tile; its values
     // are exact for the cells two or more cells away from every open face.
     // The two shell passes then overwrite the cells within two cells of an
-    // open face. rho, which the shell read for moving walls, is written by
-    // the owner of each cell alone.
+    // open face. rho, which the force pass reads for moving walls, is
+    // written by the owner of each cell alone.
     const int mode = periodic ? 0 : 1;
     if constexpr (KIND == 0) {
         hipLaunchKernelGGL((fused_step_kernel<TX, TY, ZC, true>), dim3(grid), dim3(TX * TY), 0, s, f, bc, k, pull,
