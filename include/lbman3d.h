@@ -223,6 +223,12 @@ int lbm_seed_director_field(uint64_t seed, int32_t nx, int32_t ny, int32_t nz,
  * LBM_ERR_UNSUPPORTED for wall combinations whose reference result depends on
  * write order or keeps stale populations. */
 int lbm_debug_slot_table(const lbm_config* cfg, uint16_t* table);
+/* The cells within w cells of an open face of cfg's grid, in the order the
+ * wall-shell passes of a walled step enumerate them (x, y, z per cell, each
+ * cell once). With xyz NULL only *count is set. Host-only; for tests. No
+ * counterpart in the reference: the shell passes are how this library splits
+ * a step on a grid with walls. */
+int lbm_debug_shell_region(const lbm_config* cfg, int32_t w, int32_t* xyz, int64_t max, int64_t* count);
 
 /* ---- batched form: many independent simulations in one launch per step ---- */
 typedef struct lbm_batch* lbm_batch_handle;

@@ -139,7 +139,9 @@ batch_shell_q_kernel(const SimDesc* __restrict__ descs, int nsims, int total, in
     const int w = blockIdx.x * kBlock + threadIdx.x;
     if (w >= total) return;
     const SimDesc& d = descs[find_sim(nsims, w, [&](int i) { return descs[i].sq_begin; })];
-    const Pos p = region_cell(d.r3, d.bc.n, w - d.sq_begin);
+    int q[3];
+    region_cell(d.r3, d.bc.n, w - d.sq_begin, q);
+    const Pos p{q[0], q[1], q[2]};
     const long c = cell_of(p.x, p.y, p.z, d.bc.n[0], d.bc.n[1]);
     qstep_cell(d.f[parity], d.bc, d.k, c, p, !cell_mask(d.bc, 1, p.x, p.y).writes(p.z));
 }
@@ -149,7 +151,9 @@ batch_shell_force_kernel(const SimDesc* __restrict__ descs, int nsims, int total
     const int w = blockIdx.x * kBlock + threadIdx.x;
     if (w >= total) return;
     const SimDesc& d = descs[find_sim(nsims, w, [&](int i) { return descs[i].sf_begin; })];
-    const Pos p = region_cell(d.r2, d.bc.n, w - d.sf_begin);
+    int q[3];
+    region_cell(d.r2, d.bc.n, w - d.sf_begin, q);
+    const Pos p{q[0], q[1], q[2]};
     Fields<float> fq = d.f[parity];
     for (int m = 0; m < 5; ++m) fq.q[m] = fq.qn[m];   // the force reads Q^{n+1}
     const int flags = FL_FORCE | FL_LBM | ((aux | d.aux_always) ? (FL_STORE_F | FL_STORE_RHO) : 0);

@@ -19,6 +19,7 @@
 #include "../../include/lbman3d.h"
 #include "common.hpp"
 #include "launch.hpp"
+#include "shell_region.hpp"
 #include "runtime.hpp"
 #include "solver_state.hpp"
 
@@ -977,6 +978,19 @@ int lbm_debug_slot_table(const lbm_config* cfg, uint16_t* table) {
     std::vector<uint16_t> t;
     if (int r = build_slot_table(*cfg, t)) return r;
     std::memcpy(table, t.data(), t.size() * sizeof(uint16_t));
+    return LBM_OK;
+}
+
+int lbm_debug_shell_region(const lbm_config* cfg, int32_t w, int32_t* xyz, int64_t max, int64_t* count) {
+    if (!cfg || !count || w < 1) return fail(LBM_ERR_INVALID, "bad argument");
+    if (int r = validate_config(*cfg)) return r;
+    const BCDesc<float> bc = make_bc<float>(*cfg);
+    const ShellRegion reg = make_shell_region(bc, w);
+    const long n = region_cells(reg);
+    *count = n;
+    if (!xyz) return LBM_OK;
+    if (n > max) return fail(LBM_ERR_INVALID, "xyz holds fewer than count cells");
+    for (long i = 0; i < n; ++i) region_cell(reg, bc.n, i, &xyz[3 * i]);
     return LBM_OK;
 }
 

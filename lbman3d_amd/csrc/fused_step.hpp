@@ -25,6 +25,7 @@
 
 #include "launch.hpp"
 #include "physics.hpp"
+#include "shell_region.hpp"
 #include "step_kernels.hpp"
 
 namespace lbm {
@@ -660,63 +661,6 @@ fused_step_kernel(Fields<float> f, BCDesc<float> bc, Phys<float> k, int pull, in
 }
 
 // ---- wall shells of a single simulation, one thread per cell ------------------
-// The cells within w of an open face, each once and in the class order of
-// the shells: the x band, then the y band outside the x band, then the z band
-// outside both. Along axis a the band is [0, lo) and [hi, n); its complement
-// [lo, hi) is one interval.
-struct ShellRegion {
-    int lo[3], hi[3];
-    long part[3];   // cells of the x, y and z parts
-};
-
-inline ShellRegion make_shell_region(const BCDesc<float>& bc, int w) {
-    ShellRegion r{};
-    int band[3];
-    for (int a = 0; a < 3; ++a) {
-        const int n = bc.n[a];
-        const bool open_lo = !(bc.qkind[2 * a] == QB_PERIODIC && bc.ukind[2 * a] == UB_PERIODIC);
-        const bool open_hi = !(bc.qkind[2 * a + 1] == QB_PERIODIC && bc.ukind[2 * a + 1] == UB_PERIODIC);
-        r.lo[a] = open_lo ? std::min(w, n) : 0;
-        r.hi[a] = open_hi ? std::max(r.lo[a], n - w) : n;
-        band[a] = r.lo[a] + (n - r.hi[a]);
-    }
-    const long cx = bc.n[0] - band[0], cy = bc.n[1] - band[1];
-    r.part[0] = (long)band[0] * bc.n[1] * bc.n[2];
-    r.part[1] = cx * band[1] * bc.n[2];
-    r.part[2] = cx * cy * band[2];
-    return r;
-}
-__host__ __device__ inline long region_cells(const ShellRegion& r) { return r.part[0] + r.part[1] + r.part[2]; }
-
-// Cell of item i (0 <= i < region_cells(r)); x runs fastest in every part.
-__device__ __forceinline__ Pos region_cell(const ShellRegion& r, const int n[3], long i) {
-    auto pick = [&](int a, int j) { return j < r.lo[a] ? j : r.hi[a] + (j - r.lo[a]); };
-    const int bx = r.lo[0] + (n[0] - r.hi[0]), by = r.lo[1] + (n[1] - r.hi[1]);
-    const int cx = r.hi[0] - r.lo[0], cy = r.hi[1] - r.lo[1];
-    Pos p;
-    if (i < r.part[0]) {
-        const long rest = i / bx;
-        p.x = pick(0, (int)(i - rest * bx));
-        p.y = (int)(rest % n[1]);
-        p.z = (int)(rest / n[1]);
-        return p;
-    }
-    i -= r.part[0];
-    if (i < r.part[1]) {
-        const long rest = i / cx;
-        p.x = r.lo[0] + (int)(i - rest * cx);
-        p.y = pick(1, (int)(rest % by));
-        p.z = (int)(rest / by);
-        return p;
-    }
-    i -= r.part[1];
-    const long rest = i / cx;
-    p.x = r.lo[0] + (int)(i - rest * cx);
-    p.y = r.lo[1] + (int)(rest % cy);
-    p.z = pick(2, (int)(rest / cy));
-    return p;
-}
-
 #ifndef LBM_FUSED_BODY_ONLY
 // Q update over the shell and its one-cell halo (w = 3). Every cell stores
 // Sigma and Tau for the force pass; only shell cells store Q^{n+1} and the
@@ -725,7 +669,9 @@ static __global__ void __launch_bounds__(kBlock)
 shell_qstep_kernel(Fields<float> f, BCDesc<float> bc, Phys<float> k, ShellRegion r) {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= region_cells(r)) return;
-    const Pos p = region_cell(r, bc.n, i);
+    int q[3];
+    region_cell(r, bc.n, i, q);
+    const Pos p{q[0], q[1], q[2]};
     const long c = cell_of(p.x, p.y, p.z, bc.n[0], bc.n[1]);
     qstep_cell(f, bc, k, c, p, !cell_mask(bc, 1, p.x, p.y).writes(p.z));
 }
@@ -736,7 +682,9 @@ static __global__ void __launch_bounds__(kBlock)
 shell_force_lbm_kernel(Fields<float> f, BCDesc<float> bc, Phys<float> k, ShellRegion r, int flags, int pull) {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= region_cells(r)) return;
-    const Pos p = region_cell(r, bc.n, i);
+    int q[3];
+    region_cell(r, bc.n, i, q);
+    const Pos p{q[0], q[1], q[2]};
     force_lbm_cell<float, true, true>(f, bc, k, cell_of(p.x, p.y, p.z, bc.n[0], bc.n[1]), p, flags, pull);
 }
 #endif  // LBM_FUSED_BODY_ONLY

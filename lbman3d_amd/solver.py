@@ -53,6 +53,8 @@ SIGNATURES = {
                                           C.c_double, _D]),
     "lbm_bandwidth_probe": (C.c_int, [C.c_size_t, C.c_int32, _D, _D]),
     "lbm_debug_slot_table": (C.c_int, [C.POINTER(LbmConfig), C.POINTER(C.c_uint16)]),
+    "lbm_debug_shell_region": (C.c_int, [C.POINTER(LbmConfig), C.c_int32, C.POINTER(C.c_int32), C.c_int64,
+                                         C.POINTER(C.c_int64)]),
     "lbm_batch_create": (C.c_int, [C.POINTER(LbmConfig), C.c_int32, C.POINTER(C.c_void_p)]),
     "lbm_batch_destroy": (C.c_int, [C.c_void_p]),
     "lbm_batch_upload_state": (C.c_int, [C.c_void_p, C.c_int32, _D, _D, _D, _D, _D]),
@@ -145,6 +147,16 @@ def slot_table(cfg: LbmConfig) -> np.ndarray:
     out = (C.c_uint16 * (64 * 15))()
     check(lib().lbm_debug_slot_table(C.byref(cfg), out))
     return np.array(out[:], dtype=np.uint16).reshape(64, 15)
+
+
+def shell_region(cfg: LbmConfig, w: int) -> np.ndarray:
+    """(count, 3) array of the (x, y, z) cells within w of an open face, in
+    the wall-shell passes' order (host-only)."""
+    n = C.c_int64()
+    check(lib().lbm_debug_shell_region(C.byref(cfg), w, None, 0, C.byref(n)))
+    out = (C.c_int32 * max(1, 3 * n.value))()
+    check(lib().lbm_debug_shell_region(C.byref(cfg), w, out, n.value, C.byref(n)))
+    return np.array(out[:3 * n.value], dtype=np.int32).reshape(-1, 3)
 
 
 class DeviceSolver:
