@@ -92,12 +92,19 @@ batch_step_kernel(const SimDesc* __restrict__ descs, int nsims, int total, int p
     }
 }
 
-// The same work list on the third pipeline. A simulation's advection
+// One work item of the third-pipeline kernel: planes [z0, z1) of column
+// tile txy (= tx + ty * ntx) of simulation sim.
+struct BatchItem {
+    int sim, txy, z0, z1;
+};
+
+// The work list on the third pipeline, as an explicit item list that the
+// host orders longest first (see build_items). A simulation's advection
 // scheme picks the instantiation per work item (uniform over the block).
 template <int TX, int TY, int ZC>
 __global__ void __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(2, 2)))
-batch_step_kernel_p2(const SimDesc* __restrict__ descs, int nsims, int total, int parity, int aux,
-                     unsigned* __restrict__ queue) {
+batch_step_kernel_p2(const SimDesc* __restrict__ descs, const BatchItem* __restrict__ items, int total,
+                     int parity, int aux, unsigned* __restrict__ queue) {
     using G = TileGeom<TX, TY>;
     __shared__ float sIn[2][kNumIn][G::NI];
     __shared__ float sP[2][6][G::NA];
@@ -112,14 +119,16 @@ batch_step_kernel_p2(const SimDesc* __restrict__ descs, int nsims, int total, in
         __syncthreads();
         const int w = s_item;
         if (w >= total) break;
-        const SimDesc& d = descs[find_sim(nsims, w, [&](int i) { return descs[i].tile_begin; })];
-        const int local = w - d.tile_begin;
-        const int tx = local % d.ntx, ty = (local / d.ntx) % d.nty, tz = local / (d.ntx * d.nty);
+        const BatchItem it = items[w];
+        const SimDesc& d = descs[it.sim];
+        const int tx = it.txy % d.ntx, ty = it.txy / d.ntx;
         const int a = aux | d.aux_always;
         if (d.k.advection != 0)
-            p2_tile<TX, TY, ZC, true, true, true>(d.f[parity], d.bc, d.k, d.pull, a, tx, ty, tz, sIn, sP, sRing);
+            p2_range<TX, TY, true, true, true>(d.f[parity], d.bc, d.k, d.pull, a, tx, ty, it.z0, it.z1, sIn, sP,
+                                               sRing);
         else
-            p2_tile<TX, TY, ZC, true, false, true>(d.f[parity], d.bc, d.k, d.pull, a, tx, ty, tz, sIn, sP, sRing);
+            p2_range<TX, TY, true, false, true>(d.f[parity], d.bc, d.k, d.pull, a, tx, ty, it.z0, it.z1, sIn, sP,
+                                                sRing);
         __syncthreads();
     }
     if (threadIdx.x == 0) {
@@ -173,13 +182,13 @@ int occupancy_blocks() {
 }
 
 template <int KIND, int TX, int TY, int ZC>
-void launch_batch(const SimDesc* d, int nsims, int total, int parity, int aux, int grid, unsigned* queue,
-                  hipStream_t s) {
+void launch_batch(const SimDesc* d, int nsims, int total, const BatchItem* items, int parity, int aux, int grid,
+                  unsigned* queue, hipStream_t s) {
     if constexpr (KIND == 0)
         hipLaunchKernelGGL((batch_step_kernel<TX, TY, ZC>), dim3(grid), dim3(TX * TY), 0, s, d, nsims, total,
                            parity, aux);
     else
-        hipLaunchKernelGGL((batch_step_kernel_p2<TX, TY, ZC>), dim3(grid), dim3(TX * TY), 0, s, d, nsims, total,
+        hipLaunchKernelGGL((batch_step_kernel_p2<TX, TY, ZC>), dim3(grid), dim3(TX * TY), 0, s, d, items, total,
                            parity, aux, queue);
 }
 
@@ -212,6 +221,9 @@ struct lbm_batch {
     std::vector<SimDesc> desc;
     SimDesc* d_desc = nullptr;
     unsigned* d_queue = nullptr;   // work-queue counters of the third-pipeline kernel (reset by the kernel)
+    BatchItem* d_items = nullptr;  // its item list (third-pipeline tiles only)
+    int items_cap = 0;
+    std::vector<BatchItem> items;  // host copy (the upload reads it asynchronously)
     hipStream_t stream = nullptr;
     int total = 0;
     int sq_total = 0;      // cells of the shell Q pass (simulations with walls)
@@ -255,6 +267,53 @@ int grid_for_tile(int tile, int total) {
     return std::max(1, std::min(total, cus * occ));
 }
 
+// Item list of the third-pipeline kernel: every z chunk of every column
+// tile, longest first, so that the blocks of the persistent grid end
+// together; the items of the last round (grid of them) are split into two
+// halves so that the final round is short. Sets total and grid.
+int build_items(lbm_batch* b) {
+    const BatchTile& t = kBatchTiles[b->tile];
+    std::vector<BatchItem>& items = b->items;
+    items.clear();
+    for (size_t i = 0; i < b->sims.size(); ++i) {
+        const SimDesc& d = b->desc[i];
+        const int nz = d.bc.n[2];
+        for (int txy = 0; txy < d.ntx * d.nty; ++txy)
+            for (int z0 = 0; z0 < nz; z0 += t.zc) items.push_back({(int)i, txy, z0, std::min(z0 + t.zc, nz)});
+    }
+    auto longer = [](const BatchItem& a, const BatchItem& c) { return a.z1 - a.z0 > c.z1 - c.z0; };
+    std::stable_sort(items.begin(), items.end(), longer);
+    const int grid = grid_for_tile(b->tile, (int)items.size());
+    if ((int)items.size() > 2 * grid) {
+        std::vector<BatchItem> tail(items.end() - grid, items.end());
+        items.resize(items.size() - grid);
+        for (const BatchItem& it : tail) {
+            const int mid = it.z0 + (it.z1 - it.z0) / 2;
+            if (mid > it.z0) {
+                items.push_back({it.sim, it.txy, it.z0, mid});
+                items.push_back({it.sim, it.txy, mid, it.z1});
+            } else {
+                items.push_back(it);
+            }
+        }
+        std::stable_sort(items.begin(), items.end(), longer);
+    }
+    if ((int)items.size() > b->items_cap) {
+        if (b->d_items) (void)hipFree(b->d_items);
+        b->d_items = nullptr;
+        b->items_cap = 0;
+        if (hipMalloc(&b->d_items, items.size() * sizeof(BatchItem)) != hipSuccess)
+            return fail(LBM_ERR_HIP, "hipMalloc(batch items) failed");
+        b->items_cap = (int)items.size();
+    }
+    if (hipMemcpyAsync(b->d_items, items.data(), items.size() * sizeof(BatchItem), hipMemcpyHostToDevice,
+                       b->stream) != hipSuccess)
+        return fail(LBM_ERR_HIP, "batch item upload failed");
+    b->total = (int)items.size();
+    b->grid = grid_for_tile(b->tile, b->total);
+    return LBM_OK;
+}
+
 // Build the descriptors from the handles' current buffers (parity 0 view).
 int rebuild(lbm_batch* b) {
     b->desc.resize(b->sims.size());
@@ -285,6 +344,8 @@ int rebuild(lbm_batch* b) {
     }
     b->total = begin;
     b->grid = grid_for_tile(b->tile, b->total);
+    if (kBatchTiles[b->tile].kind == 1)
+        if (int r = build_items(b)) return r;
     b->sq_total = sq_begin;
     b->sf_total = sf_begin;
     if (hipMemcpyAsync(b->d_desc, b->desc.data(), b->desc.size() * sizeof(SimDesc), hipMemcpyHostToDevice,
@@ -316,8 +377,8 @@ void launch_step(lbm_batch* b, int aux) {
     const int n = (int)b->sims.size();
     with_batch_tile(b->tile, [&](auto t) {
         using T = decltype(t);
-        launch_batch<T::kind, T::tx, T::ty, T::zc>(b->d_desc, n, b->total, b->parity, aux, b->grid, b->d_queue,
-                                                   b->stream);
+        launch_batch<T::kind, T::tx, T::ty, T::zc>(b->d_desc, n, b->total, b->d_items, b->parity, aux, b->grid,
+                                                   b->d_queue, b->stream);
     });
     if (b->sf_total > 0) {
         hipLaunchKernelGGL(batch_shell_q_kernel, dim3(grid_for(b->sq_total)), dim3(kBlock), 0, b->stream, b->d_desc,
@@ -432,6 +493,7 @@ int lbm_batch_destroy(lbm_batch_handle b) {
     drop_graphs(b);
     for (lbm_handle h : b->sims) lbm_destroy(h);
     if (b->d_desc) (void)hipFree(b->d_desc);
+    if (b->d_items) (void)hipFree(b->d_items);
     if (b->d_queue) (void)hipFree(b->d_queue);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;

@@ -951,19 +951,18 @@ __device__ __forceinline__ void p_sets(float alpha, const StageA& a, float pxy[6
 //   after it: Q update at plane pa, then force + collide-stream at plane pa - 1.
 // A wave can run at most one iteration ahead of the slowest (the next
 // barrier), which the two LDS buffers and the four ring-input slots cover.
-// One call computes z planes [tz*ZC, tz*ZC + ZC) of the TX x TY column tile
-// (tx, ty), with LDS buffers from the calling kernel; a kernel that runs
-// several tiles per block puts a barrier between them.
-template <int TX, int TY, int ZC, bool MASKED, bool UPWIND, bool NT>
-__device__ __forceinline__ void p2_tile(const Fields<float>& f, const BCDesc<float>& bc, const Phys<float>& k,
-                                       int pull, int aux, int tx, int ty, int tz,
-                                       float (*sIn)[kNumIn][TileGeom<TX, TY>::NI],
-                                       float (*sP)[6][TileGeom<TX, TY>::NA],
-                                       float (*sRing)[kNumIn][TileGeom<TX, TY>::NR]) {
+// One call computes z planes [z0, z1) (0 <= z0 < z1 <= nz) of the TX x TY
+// column tile (tx, ty), with LDS buffers from the calling kernel; a kernel
+// that runs several tiles per block puts a barrier between them.
+template <int TX, int TY, bool MASKED, bool UPWIND, bool NT>
+__device__ __forceinline__ void p2_range(const Fields<float>& f, const BCDesc<float>& bc, const Phys<float>& k,
+                                        int pull, int aux, int tx, int ty, int z0, int z1,
+                                        float (*sIn)[kNumIn][TileGeom<TX, TY>::NI],
+                                        float (*sP)[6][TileGeom<TX, TY>::NA],
+                                        float (*sRing)[kNumIn][TileGeom<TX, TY>::NR]) {
     using G = TileGeom<TX, TY>;
     const int nx = bc.n[0], ny = bc.n[1], nz = bc.n[2];
     const int x0 = tx * TX, y0 = ty * TY;
-    const int z0 = tz * ZC, z1 = min(z0 + ZC, nz);
     const unsigned plane = (unsigned)nx * ny;
 
     const int t = threadIdx.x;
@@ -1134,6 +1133,17 @@ __device__ __forceinline__ void p2_tile(const Fields<float>& f, const BCDesc<flo
             for (int m = 0; m < kNumIn; ++m) ov[m] = onext[m];
         }
     }
+}
+
+// Chunk tz of ZC planes of the column tile (tx, ty).
+template <int TX, int TY, int ZC, bool MASKED, bool UPWIND, bool NT>
+__device__ __forceinline__ void p2_tile(const Fields<float>& f, const BCDesc<float>& bc, const Phys<float>& k,
+                                       int pull, int aux, int tx, int ty, int tz,
+                                       float (*sIn)[kNumIn][TileGeom<TX, TY>::NI],
+                                       float (*sP)[6][TileGeom<TX, TY>::NA],
+                                       float (*sRing)[kNumIn][TileGeom<TX, TY>::NR]) {
+    const int z0 = tz * ZC;
+    p2_range<TX, TY, MASKED, UPWIND, NT>(f, bc, k, pull, aux, tx, ty, z0, min(z0 + ZC, bc.n[2]), sIn, sP, sRing);
 }
 
 template <int TX, int TY, int ZC, bool MASKED, int WPE, bool UPWIND, bool NT, bool ZSLOW>
