@@ -230,6 +230,13 @@ int lbm_debug_slot_table(const lbm_config* cfg, uint16_t* table);
  * a step on a grid with walls. */
 int lbm_debug_shell_region(const lbm_config* cfg, int32_t w, int32_t* xyz, int64_t max, int64_t* count);
 
+/* How the fused step maps a coordinate t outside [0, n) back into the domain:
+ * kind 0 for z planes (requested from two cells below a face to one above
+ * it), kind 1 for the x and y columns of a tile (any distance); a periodic
+ * side wraps, any other side clamps. Host-only; for tests. No counterpart in
+ * the reference: the tiles of the fused step are this library's own. */
+int lbm_debug_tile_map(int32_t kind, int32_t t, int32_t n, int32_t lo_per, int32_t hi_per, int32_t* out);
+
 /* ---- batched form: many independent simulations in one launch per step ---- */
 typedef struct lbm_batch* lbm_batch_handle;
 /* All configs must be LBM_FP32_SHIFTED. */
@@ -249,9 +256,15 @@ int lbm_batch_diagnostics(lbm_batch_handle b, int32_t sim, lbm_diag* out);
  * that times every shape on scratch copies of the states and keeps the
  * fastest (ms per step in ms_out[count]). */
 int lbm_batch_tile_count(void);
+int lbm_batch_tile_name(int32_t tile, char* buf, size_t len);
 int lbm_batch_set_tile(lbm_batch_handle b, int32_t tile);
 int lbm_batch_get_tile(lbm_batch_handle b, int32_t* tile);
 int lbm_batch_autotune(lbm_batch_handle b, int32_t n, double* ms_out, int32_t max, int32_t* count);
+/* Cap the persistent grid of the batched kernels at max_blocks blocks (0, the
+ * default: as many as fit on the device at once). With a grid smaller than
+ * the work list, blocks take several tiles and the final round of the work
+ * list is split; results do not depend on the cap. For tests and tuning. */
+int lbm_batch_set_grid_limit(lbm_batch_handle b, int32_t max_blocks);
 
 /* ---- learned closure for a 2x-coarsened FP32 simulation -------------------
  * No counterpart in the reference. A periodic 3x3x3 convolutional network

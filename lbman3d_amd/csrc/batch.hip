@@ -231,6 +231,7 @@ struct lbm_batch {
     int parity = 0;        // how far the buffers have ping-ponged since the handles were synced
     int tile = 0;
     int grid = 0;
+    int grid_limit = 0;    // cap on the persistent grid (0: as many blocks as fit at once)
     bool desc_dirty = true;
     hipGraphExec_t graph[2] = {nullptr, nullptr};   // two steps starting at parity 0 / 1
 };
@@ -254,7 +255,7 @@ int tiles_of(const lbm_batch* b, int i, int& ntx, int& nty, int& nzc) {
     return ntx * nty * nzc;
 }
 
-int grid_for_tile(int tile, int total) {
+int grid_for_tile(int tile, int total, int limit) {
     int dev = 0, cus = 256;
     if (hipGetDevice(&dev) == hipSuccess) {
         hipDeviceProp_t p;
@@ -264,7 +265,8 @@ int grid_for_tile(int tile, int total) {
         using T = decltype(t);
         return occupancy_blocks<T::kind, T::tx, T::ty, T::zc>();
     });
-    return std::max(1, std::min(total, cus * occ));
+    const int grid = std::max(1, std::min(total, cus * occ));
+    return limit > 0 ? std::min(grid, limit) : grid;
 }
 
 // Item list of the third-pipeline kernel: every z chunk of every column
@@ -283,7 +285,7 @@ int build_items(lbm_batch* b) {
     }
     auto longer = [](const BatchItem& a, const BatchItem& c) { return a.z1 - a.z0 > c.z1 - c.z0; };
     std::stable_sort(items.begin(), items.end(), longer);
-    const int grid = grid_for_tile(b->tile, (int)items.size());
+    const int grid = grid_for_tile(b->tile, (int)items.size(), b->grid_limit);
     if ((int)items.size() > 2 * grid) {
         std::vector<BatchItem> tail(items.end() - grid, items.end());
         items.resize(items.size() - grid);
@@ -310,7 +312,7 @@ int build_items(lbm_batch* b) {
                        b->stream) != hipSuccess)
         return fail(LBM_ERR_HIP, "batch item upload failed");
     b->total = (int)items.size();
-    b->grid = grid_for_tile(b->tile, b->total);
+    b->grid = grid_for_tile(b->tile, b->total, b->grid_limit);
     return LBM_OK;
 }
 
@@ -343,7 +345,7 @@ int rebuild(lbm_batch* b) {
         sf_begin += (int)region_cells(d.r2);
     }
     b->total = begin;
-    b->grid = grid_for_tile(b->tile, b->total);
+    b->grid = grid_for_tile(b->tile, b->total, b->grid_limit);
     if (kBatchTiles[b->tile].kind == 1)
         if (int r = build_items(b)) return r;
     b->sq_total = sq_begin;
@@ -548,6 +550,14 @@ int lbm_batch_time_steps(lbm_batch_handle b, int32_t n, int32_t use_graph, doubl
 
 int lbm_batch_tile_count(void) { return kNumBatchTiles; }
 
+int lbm_batch_tile_name(int32_t tile, char* buf, size_t len) {
+    if (!buf || !len) return fail(LBM_ERR_INVALID, "null buffer");
+    if (tile < 0 || tile >= kNumBatchTiles) return fail(LBM_ERR_INVALID, "tile out of range");
+    std::strncpy(buf, kBatchTiles[tile].name, len - 1);
+    buf[len - 1] = 0;
+    return LBM_OK;
+}
+
 int lbm_batch_set_tile(lbm_batch_handle b, int32_t tile) {
     if (!b || tile < 0 || tile >= kNumBatchTiles) return fail(LBM_ERR_INVALID, "tile out of range");
     if (int r = sync_handles(b)) return r;
@@ -594,6 +604,14 @@ int lbm_batch_autotune(lbm_batch_handle b, int32_t n, double* ms_out, int32_t ma
     }
     b->desc_dirty = true;
     if (count) *count = std::min(kNumBatchTiles, (int)max);
+    return LBM_OK;
+}
+
+int lbm_batch_set_grid_limit(lbm_batch_handle b, int32_t max_blocks) {
+    if (!b || max_blocks < 0) return fail(LBM_ERR_INVALID, "bad argument");
+    if (int r = sync_handles(b)) return r;
+    b->grid_limit = max_blocks;
+    b->desc_dirty = true;
     return LBM_OK;
 }
 

@@ -20,6 +20,7 @@
 #include "common.hpp"
 #include "launch.hpp"
 #include "shell_region.hpp"
+#include "tile_map.hpp"
 #include "runtime.hpp"
 #include "solver_state.hpp"
 
@@ -991,6 +992,12 @@ int lbm_debug_shell_region(const lbm_config* cfg, int32_t w, int32_t* xyz, int64
     if (!xyz) return LBM_OK;
     if (n > max) return fail(LBM_ERR_INVALID, "xyz holds fewer than count cells");
     for (long i = 0; i < n; ++i) region_cell(reg, bc.n, i, &xyz[3 * i]);
+    return LBM_OK;
+}
+
+int lbm_debug_tile_map(int32_t kind, int32_t t, int32_t n, int32_t lo_per, int32_t hi_per, int32_t* out) {
+    if (!out || n < 1 || (kind != 0 && kind != 1)) return fail(LBM_ERR_INVALID, "bad argument");
+    *out = kind == 0 ? map_coord(t, n, lo_per != 0, hi_per != 0) : map_col(t, n, lo_per != 0, hi_per != 0);
     return LBM_OK;
 }
 

@@ -26,43 +26,10 @@
 #include "launch.hpp"
 #include "physics.hpp"
 #include "shell_region.hpp"
+#include "tile_map.hpp"
 #include "step_kernels.hpp"
 
 namespace lbm {
-
-// Map a coordinate of the extended tile into the domain: periodic side wraps,
-// any other side clamps (the rule of nb_index for arbitrary distance).
-// map_coord serves z planes, which lie at most two cells past a face (one
-// addition or subtraction of n, or two for n = 1). map_col serves the x and y
-// columns, fixed per thread: the inactive columns of a tile wider than the
-// domain can lie further out and wrap by a remainder, so that every load
-// stays inside the domain.
-__device__ __forceinline__ int map_coord(int t, int n, bool lo_per, bool hi_per) {
-    if (t < 0) {
-        if (!lo_per) return 0;
-        t += n;
-        return t < 0 ? t + n : t;
-    }
-    if (t >= n) {
-        if (!hi_per) return n - 1;
-        t -= n;
-        return t >= n ? t - n : t;
-    }
-    return t;
-}
-__device__ __forceinline__ int map_col(int t, int n, bool lo_per, bool hi_per) {
-    if (t < 0) {
-        if (!lo_per) return 0;
-        t += n;
-        return t >= 0 ? t : (n - 1) - ((-t - 1) % n);
-    }
-    if (t >= n) {
-        if (!hi_per) return n - 1;
-        t -= n;
-        return t < n ? t : t % n;
-    }
-    return t;
-}
 
 // Global access as uniform base + 32-bit byte offset, so the compiler can use
 // the scalar-base addressing form (one VGPR per address instead of two).
@@ -787,8 +754,11 @@ fused_step_kernel_pf(Fields<float> f, BCDesc<float> bc, Phys<float> k, int pull,
         const bool do_b = pb >= z0 && active;
         // ---- step 1: issue next-plane loads and the pull gather; stage plane pa ----
         float hnext[kNumIn], rnext[kNumIn], onext[kNumIn], pop[kQ];
-        load_column<TX, TY>(f, qc0, uc0, qplane(pa + 2), uplane(pa + 2), hnext);
-        if (has_ring) load_column<TX, TY>(f, qcr, ucr, qplane(pa + 2), uplane(pa + 2), rnext);
+        // At pa == z1 the loaded plane is never used; clamping it to z1 + 1
+        // keeps every request within one plane past the chunk.
+        const int pn = min(pa + 2, z1 + 1);
+        load_column<TX, TY>(f, qc0, uc0, qplane(pn), uplane(pn), hnext);
+        if (has_ring) load_column<TX, TY>(f, qcr, ucr, qplane(pn), uplane(pn), rnext);
         if (has_outer) load_column<TX, TY>(f, oqc, ouc, qplane(pa + 1), uplane(pa + 1), onext);
         const unsigned cb = (unsigned)pb * plane + (unsigned)gy * nx + gx;
         if (do_b) gather32<PERIODIC>(f, bc, gx, gy, pb, cb, pull, pop);
@@ -1022,8 +992,9 @@ __device__ __forceinline__ void p2_range(const Fields<float>& f, const BCDesc<fl
         const bool do_b = pb >= z0 && active;
         // ---- before the barrier: loads in flight, publish plane pa - 1, stage plane pa ----
         float hnext[kNumIn], rnext[kNumIn], onext[kNumIn], pop[kQ];
-        load_column<TX, TY>(f, c0, c0, zplane(pa + 2), zplane(pa + 2), hnext);
-        if (has_ring) load_column<TX, TY>(f, cr, cr, zplane(pa + 2), zplane(pa + 2), rnext);
+        const int pn = min(pa + 2, z1 + 1);   // unused at pa == z1, as in fused_step_kernel_pf
+        load_column<TX, TY>(f, c0, c0, zplane(pn), zplane(pn), hnext);
+        if (has_ring) load_column<TX, TY>(f, cr, cr, zplane(pn), zplane(pn), rnext);
         if (has_outer) load_column<TX, TY>(f, oc, oc, zplane(pa + 1), zplane(pa + 1), onext);
         const unsigned cb = (unsigned)pb * plane + (unsigned)gy * nx + gx;
         if (do_b) gather32<true>(f, bc, gx, gy, pb, cb, pull, pop);

@@ -5,9 +5,7 @@ Follows the reference's DefectFinder (src/analysis/defect_finder.tpp):
   1. ComputeWindingNumbers: the plaquettes a disclination pierces
      (lbm_defect_faces, computed on the GPU);
   2. BuildConnectivityGraph (:164-340): pierced plaquettes that share a
-     voxel are joined, each voxel contributing the edges between its
-     "forward" plaquettes and two/one backward ones, with Q-tag offsets at
-     the domain faces (periodic wrap or Neumann clamp, src/offsets.h:59-83);
+     voxel are joined (the edge set is stated per voxel in DefectGraph);
   3. IsolateDisclinationsFromGraph (:342-356) with the graph's FindPaths
      (src/analysis/defect_connectivity_graph.h:72-110): the edge set is
      walked into paths, each started at an odd-degree node when there is
@@ -29,7 +27,7 @@ import os
 
 import numpy as np
 
-from . import abi, vtkhdf
+from . import vtkhdf
 from .solver import check, lib
 
 VTK_LINE = 3
@@ -46,31 +44,34 @@ def device_defect_faces(sim):
     return tuple(out)
 
 
-def _axis_offset(i, d, n, lo_periodic, hi_periodic):
-    """QAxisOffset: i + d with a periodic side wrapping and any other side
-    clamping (src/offsets.h:59-83)."""
-    j = i + d
-    if j < 0:
-        return j + n if lo_periodic else 0
-    if j >= n:
-        return j - n if hi_periodic else n - 1
-    return j
-
-
 class DefectGraph:
     """The connectivity graph of pierced plaquettes (face ids as in
-    DefectFields: x faces first, then y, then z)."""
+    DefectFields: x faces first, then y, then z).
 
-    def __init__(self, nx, ny, nz, def_x, def_y, def_z, q_periodic=(True,) * 6):
+    Edges, by voxel: a voxel (x, y, z), 0 <= x < nx-1 (likewise y, z), has
+    six plaquette faces, a low and a high one along each axis. Every two
+    pierced faces of one voxel are joined, except that the pairs of high
+    faces (x-high with y-high, x-high with z-high) are joined only where
+    x < nx-2, and the pair (y-high, z-high) only where y < ny-2. This is the
+    edge set that BuildConnectivityGraph (src/analysis/defect_finder.tpp:
+    164-340) produces: it adds the high-high pairs of a voxel when it visits
+    the next voxel along x or y, which does not exist in the last layer, and
+    it visits no voxel across a periodic seam (its wrapped backward offsets
+    are skipped by the `xm < nx-1` and `ym < ny-1` guards). The graph
+    therefore does not depend on the Q boundary tags."""
+
+    # the 15 face pairs of a voxel; faces 0..5 = x-lo, x-hi, y-lo, y-hi, z-lo, z-hi
+    _PAIRS = [(a, b) for a in range(6) for b in range(a + 1, 6)]
+
+    def __init__(self, nx, ny, nz, def_x, def_y, def_z):
         self.n = (nx, ny, nz)
         self.nfx = nx * (ny - 1) * (nz - 1)
         self.nfy = (nx - 1) * ny * (nz - 1)
         self.dx, self.dy, self.dz = (np.asarray(a, dtype=np.uint8) for a in (def_x, def_y, def_z))
-        self.qp = q_periodic
         self.adj: dict[int, set] = {}
         self._build()
 
-    # flat indices (DefectFields::FlatX/Y/Z) and face ids (FidX/Y/Z)
+    # flat indices (DefectFields::FlatX/Y/Z)
     def fx(self, x, y, z):
         nx, ny, _ = self.n
         return (z * (ny - 1) + y) * nx + x
@@ -83,82 +84,36 @@ class DefectGraph:
         nx, ny, _ = self.n
         return (z * (ny - 1) + y) * (nx - 1) + x
 
-    def _edge(self, a, b):
-        if a == b:
-            return
-        self.adj.setdefault(a, set()).add(b)
-        self.adj.setdefault(b, set()).add(a)
-
-    def _voxel(self, x, y, z, xm, xp, ym, yp, zp):
-        nx, ny, _ = self.n
-        DX, DY, DZ = self.dx, self.dy, self.dz
-        idx = lambda f: f
-        iy = lambda f: self.nfx + f
-        iz = lambda f: self.nfx + self.nfy + f
-        if DX[self.fx(x, y, z)]:
-            first = idx(self.fx(x, y, z))
-            if DZ[self.fz(x, y, z)]:
-                self._edge(first, iz(self.fz(x, y, z)))
-            if DZ[self.fz(x, y, zp)]:
-                self._edge(first, iz(self.fz(x, y, zp)))
-            if DY[self.fy(x, y, z)]:
-                self._edge(first, iy(self.fy(x, y, z)))
-            if DY[self.fy(x, yp, z)]:
-                self._edge(first, iy(self.fy(x, yp, z)))
-            if DX[self.fx(xp, y, z)]:
-                self._edge(first, idx(self.fx(xp, y, z)))
-            if xm < nx - 1 and DZ[self.fz(xm, y, zp)]:
-                self._edge(first, iz(self.fz(xm, y, zp)))
-            if xm < nx - 1 and DY[self.fy(xm, yp, z)]:
-                self._edge(first, iy(self.fy(xm, yp, z)))
-        if DY[self.fy(x, y, z)]:
-            first = iy(self.fy(x, y, z))
-            if DZ[self.fz(x, y, z)]:
-                self._edge(first, iz(self.fz(x, y, z)))
-            if DZ[self.fz(x, y, zp)]:
-                self._edge(first, iz(self.fz(x, y, zp)))
-            if DX[self.fx(xp, y, z)]:
-                self._edge(first, idx(self.fx(xp, y, z)))
-            if DY[self.fy(x, yp, z)]:
-                self._edge(first, iy(self.fy(x, yp, z)))
-            if ym < ny - 1 and DZ[self.fz(x, ym, zp)]:
-                self._edge(first, iz(self.fz(x, ym, zp)))
-        if DZ[self.fz(x, y, z)]:
-            first = iz(self.fz(x, y, z))
-            if DX[self.fx(xp, y, z)]:
-                self._edge(first, idx(self.fx(xp, y, z)))
-            if DY[self.fy(x, yp, z)]:
-                self._edge(first, iy(self.fy(x, yp, z)))
-            if DZ[self.fz(x, y, zp)]:
-                self._edge(first, iz(self.fz(x, y, zp)))
+    def edges(self) -> set:
+        """Every edge once, as (smaller face id, larger face id)."""
+        return {(a, b) for a, nb in self.adj.items() for b in nb if a < b}
 
     def _build(self):
         nx, ny, nz = self.n
-        p = self.qp
-        off = lambda i, d, n, a: _axis_offset(i, d, n, p[2 * a], p[2 * a + 1])
-        if not (self.dx.any() or self.dy.any() or self.dz.any()):
+        if min(nx, ny, nz) < 2:
             return
-        # interior voxels, then the six boundary layers with face offsets
-        for z in range(1, nz - 2):
-            for y in range(1, ny - 2):
-                for x in range(1, nx - 2):
-                    self._voxel(x, y, z, x - 1, x + 1, y - 1, y + 1, z + 1)
-
-        def bvox(x, y, z):
-            self._voxel(x, y, z, off(x, -1, nx, 0), off(x, 1, nx, 0), off(y, -1, ny, 1), off(y, 1, ny, 1),
-                        off(z, 1, nz, 2))
-        for z in sorted({0, nz - 2}):
-            for y in range(ny - 1):
-                for x in range(nx - 1):
-                    bvox(x, y, z)
-        for z in range(nz - 1):
-            for y in sorted({0, ny - 2}):
-                for x in range(nx - 1):
-                    bvox(x, y, z)
-        for z in range(nz - 1):
-            for y in range(ny - 1):
-                for x in sorted({0, nx - 2}):
-                    bvox(x, y, z)
+        DX = self.dx.reshape(nz - 1, ny - 1, nx).astype(bool)
+        DY = self.dy.reshape(nz - 1, ny, nx - 1).astype(bool)
+        DZ = self.dz.reshape(nz, ny - 1, nx - 1).astype(bool)
+        ix = np.arange(self.nfx).reshape(nz - 1, ny - 1, nx)
+        iy = self.nfx + np.arange(self.nfy).reshape(nz - 1, ny, nx - 1)
+        iz = self.nfx + self.nfy + np.arange((nx - 1) * (ny - 1) * nz).reshape(nz, ny - 1, nx - 1)
+        # the six faces of every voxel, as (pierced, id) arrays of shape (nz-1, ny-1, nx-1)
+        faces = [(DX[:, :, :-1], ix[:, :, :-1]), (DX[:, :, 1:], ix[:, :, 1:]),
+                 (DY[:, :-1, :], iy[:, :-1, :]), (DY[:, 1:, :], iy[:, 1:, :]),
+                 (DZ[:-1], iz[:-1]), (DZ[1:], iz[1:])]
+        last_x = np.zeros((nz - 1, ny - 1, nx - 1), dtype=bool)
+        last_x[:, :, -1] = True
+        last_y = np.zeros_like(last_x)
+        last_y[:, -1, :] = True
+        skip = {(1, 3): last_x, (1, 5): last_x, (3, 5): last_y}
+        for a, b in self._PAIRS:
+            m = faces[a][0] & faces[b][0]
+            if (a, b) in skip:
+                m = m & ~skip[(a, b)]
+            for u, v in zip(faces[a][1][m].tolist(), faces[b][1][m].tolist()):
+                self.adj.setdefault(u, set()).add(v)
+                self.adj.setdefault(v, set()).add(u)
 
     def paths(self):
         """FindPaths: walks consuming every edge once."""
@@ -242,8 +197,7 @@ def find_disclinations(sim):
     """Disclination polylines of a DeviceSolver's current state."""
     cfg = sim.cfg
     faces = device_defect_faces(sim)
-    qp = tuple(cfg.walls[f].qbc == abi.Q_PERIODIC for f in range(6))
-    return DefectGraph(cfg.nx, cfg.ny, cfg.nz, *faces, q_periodic=qp).disclinations()
+    return DefectGraph(cfg.nx, cfg.ny, cfg.nz, *faces).disclinations()
 
 
 def export_disclinations(sim, directory: str, frame: int, num_steps: int) -> str:

@@ -55,6 +55,7 @@ SIGNATURES = {
     "lbm_debug_slot_table": (C.c_int, [C.POINTER(LbmConfig), C.POINTER(C.c_uint16)]),
     "lbm_debug_shell_region": (C.c_int, [C.POINTER(LbmConfig), C.c_int32, C.POINTER(C.c_int32), C.c_int64,
                                          C.POINTER(C.c_int64)]),
+    "lbm_debug_tile_map": (C.c_int, [C.c_int32] * 5 + [C.POINTER(C.c_int32)]),
     "lbm_batch_create": (C.c_int, [C.POINTER(LbmConfig), C.c_int32, C.POINTER(C.c_void_p)]),
     "lbm_batch_destroy": (C.c_int, [C.c_void_p]),
     "lbm_batch_upload_state": (C.c_int, [C.c_void_p, C.c_int32, _D, _D, _D, _D, _D]),
@@ -63,9 +64,11 @@ SIGNATURES = {
     "lbm_batch_time_steps": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, _D]),
     "lbm_batch_diagnostics": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(LbmDiag)]),
     "lbm_batch_tile_count": (C.c_int, []),
+    "lbm_batch_tile_name": (C.c_int, [C.c_int32, C.c_char_p, C.c_size_t]),
     "lbm_batch_set_tile": (C.c_int, [C.c_void_p, C.c_int32]),
     "lbm_batch_get_tile": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32)]),
     "lbm_batch_autotune": (C.c_int, [C.c_void_p, C.c_int32, _D, C.c_int32, C.POINTER(C.c_int32)]),
+    "lbm_batch_set_grid_limit": (C.c_int, [C.c_void_p, C.c_int32]),
     "lbm_closure_create": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_uint64,
                                      C.POINTER(C.c_void_p)]),
     "lbm_closure_destroy": (C.c_int, [C.c_void_p]),
@@ -271,6 +274,20 @@ def tile_count() -> int:
     return lib().lbm_tile_count()
 
 
+def batch_tile_name(t: int) -> str:
+    buf = C.create_string_buffer(64)
+    check(lib().lbm_batch_tile_name(t, buf, 64))
+    return buf.value.decode()
+
+
+def batch_tile_index(name: str) -> int:
+    """Catalogue index of a batched tile shape by name."""
+    for t in range(lib().lbm_batch_tile_count()):
+        if batch_tile_name(t) == name:
+            return t
+    raise KeyError(name)
+
+
 def schedule_name(s: int) -> str:
     buf = C.create_string_buffer(128)
     check(lib().lbm_tile_name(s, buf, 128))
@@ -339,6 +356,10 @@ class BatchSolver:
         t = C.c_int32()
         check(lib().lbm_batch_get_tile(self.h, C.byref(t)))
         return t.value
+
+    def set_grid_limit(self, max_blocks: int):
+        """Cap the persistent grid (0: automatic)."""
+        check(lib().lbm_batch_set_grid_limit(self.h, max_blocks))
 
     def autotune(self, n=5) -> list:
         ms = (C.c_double * 16)()

@@ -10,6 +10,8 @@ src/vtkhdf_writer.h:115-160):
                                Origin = {0, 0, 0}, Spacing = {1, 1, 1},
                                Direction = identity (9 doubles)
     /VTKHDF/PointData/rho      float64 [nz, ny, nx]
+    /VTKHDF/PointData/f0..f14  float64 [nz, ny, nx]   populations, debug frames
+                                                      only (Params::kDebugLogging)
     /VTKHDF/PointData/order    float64 [nz, ny, nx]   largest eigenvalue of Q
     /VTKHDF/PointData/ux|uy|uz float64 [nz, ny, nx]
     /VTKHDF/PointData/director float64 [nz, ny, nx, 3]
@@ -21,8 +23,8 @@ that this layout needs: a version-0 superblock, symbol-table groups (local
 heap, version-1 B-tree of one leaf, one symbol-table node), version-1 object
 headers, contiguous datasets and attributes of fixed-point, IEEE double and
 fixed-length string type. ``read_h5`` is an independent reader of the same
-subset used by the tests; no HDF5 library was available to check the files
-against.
+subset used by the tests, which also read every file with the HDF Group's
+h5dump where the machine has it.
 """
 from __future__ import annotations
 
@@ -33,7 +35,10 @@ import numpy as np
 
 UNDEF = 0xFFFFFFFFFFFFFFFF
 SIG = b"\x89HDF\r\n\x1a\n"
-LEAF_K, INTERNAL_K = 4, 16          # symbol-table node holds 2*LEAF_K entries
+# A symbol-table node holds 2*LEAF_K entries, and every group here is one
+# node: K = 16 (libhdf5's default is 4; the superblock records the value)
+# leaves room for the 21 members of a debug frame's PointData group.
+LEAF_K, INTERNAL_K = 16, 16
 FREE_NULL = 1                       # end of a local heap's free list on disk
 
 
@@ -352,9 +357,11 @@ def _read_group(data, btree, heap, leaf_k):
 
 # ---- the reference's frames ---------------------------------------------------
 
-def image_data_tree(nx, ny, nz, rho, ux, uy, uz, order, director) -> H5File:
+def image_data_tree(nx, ny, nz, rho, ux, uy, uz, order, director, populations=None) -> H5File:
     """The ImageData frame of SimIO::ExportVTKHDF (fields in cell order
-    (z*ny + y)*nx + x; director [V][3])."""
+    (z*ny + y)*nx + x; director [V][3]). With populations (15 blocks of V,
+    f_i of every cell) the frame also holds the datasets f0..f14 that the
+    reference writes when Params::kDebugLogging is set (src/sim_io.cc:185-196)."""
     f = H5File()
     vt = f.root.group("VTKHDF", {
         "Type": "ImageData",
@@ -366,7 +373,12 @@ def image_data_tree(nx, ny, nz, rho, ux, uy, uz, order, director) -> H5File:
     })
     pd = vt.group("PointData")
     shape = (nz, ny, nx)
-    for name, arr in (("rho", rho), ("order", order), ("ux", ux), ("uy", uy), ("uz", uz)):
+    pd.dataset("rho", np.asarray(rho, dtype=np.float64).reshape(shape))
+    if populations is not None:
+        pops = np.asarray(populations, dtype=np.float64).reshape(15, -1)
+        for i in range(15):
+            pd.dataset(f"f{i}", pops[i].reshape(shape))
+    for name, arr in (("order", order), ("ux", ux), ("uy", uy), ("uz", uz)):
         pd.dataset(name, np.asarray(arr, dtype=np.float64).reshape(shape))
     pd.dataset("director", np.asarray(director, dtype=np.float64).reshape(shape + (3,)))
     return f
@@ -381,12 +393,16 @@ def step_width(num_steps: int) -> int:
     return w
 
 
-def export_frame(sim, directory: str, frame: int, num_steps: int) -> str:
-    """Write ``<directory>/lbm_<frame>.vtkhdf`` from a device solver's state."""
+def export_frame(sim, directory: str, frame: int, num_steps: int, debug: bool = False) -> str:
+    """Write ``<directory>/lbm_<frame>.vtkhdf`` from a device solver's state;
+    ``debug`` adds the populations f0..f14 (the post-streaming f of the last
+    step, the host populations the reference's CPU path writes)."""
     cfg = sim.cfg
     st = sim.state()
     order, director = sim.order_director()
-    f = image_data_tree(cfg.nx, cfg.ny, cfg.nz, st["rho"], st["u"][0], st["u"][1], st["u"][2], order, director)
+    pops = sim.populations() if debug else None
+    f = image_data_tree(cfg.nx, cfg.ny, cfg.nz, st["rho"], st["u"][0], st["u"][1], st["u"][2], order, director,
+                        pops)
     os.makedirs(directory, exist_ok=True)
     path = os.path.join(directory, f"lbm_{frame:0{step_width(num_steps)}d}.vtkhdf")
     f.save(path)

@@ -56,3 +56,29 @@ def test_run_mode_matches_python_path_and_oracle(tmp_path):
     for k in ("rho", "u", "q"):
         np.testing.assert_array_equal(got[k], py[k])
         np.testing.assert_array_equal(got[k], ref[k])
+
+
+REFERENCE_SRC = "/root/reference/src"
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(REFERENCE_SRC, "boundary.h")),
+                    reason="the reference sources are not present on this machine")
+def test_binding_compiles_against_the_reference_headers(tmp_path):
+    """include/lbman3d.hpp plus the five trait lines of the integration guide,
+    compiled against the reference's real boundary.h, params.h and
+    sim_config.h (not stand-ins): every reference preset, SimBC, the default
+    BCConfig and a custom config with Anchoring and MovingWall lower to the
+    configs the library builds, and params.h lowers to its defaults."""
+    exe = tmp_path / "ref_boundary_check"
+    src = os.path.join(ROOT, "tests", "cpp", "ref_boundary_check.cpp")
+    lib_dir = os.path.join(ROOT, "lbman3d_amd")
+    if not os.path.exists(os.path.join(lib_dir, "liblbman3d.so")):
+        from lbman3d_amd import build
+        build.build()
+    cmd = ["g++", "-std=c++20", "-O1", "-Wall", "-I", REFERENCE_SRC, "-I", os.path.join(ROOT, "include"), src,
+           "-L", lib_dir, "-llbman3d", f"-Wl,-rpath,{lib_dir}", "-o", str(exe)]
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "all checks passed" in r.stdout

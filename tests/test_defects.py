@@ -103,6 +103,36 @@ def test_graph_closes_a_loop():
     assert all(len(v) == 2 for v in g.adj.values()) and len(g.adj) == 4
 
 
+@pytest.mark.parametrize("dims", [(2, 2, 2), (3, 2, 4), (5, 4, 3), (6, 7, 5), (9, 8, 7)])
+@pytest.mark.parametrize("density", [0.08, 0.3, 0.7])
+def test_voxel_pair_graph_equals_the_reference_visit(dims, density):
+    """The graph built per voxel (pairs of pierced faces) has exactly the
+    edges of the reference's visit order (oracle restatement of
+    BuildConnectivityGraph) for every combination of periodic and clamped Q
+    faces, on random pierced sets from sparse to dense."""
+    nx, ny, nz = dims
+    rng = np.random.default_rng(nx * 100 + ny * 10 + nz + int(density * 1000))
+    sizes = (nx * (ny - 1) * (nz - 1), (nx - 1) * ny * (nz - 1), (nx - 1) * (ny - 1) * nz)
+    faces = [(rng.random(n) < density).astype(np.uint8) for n in sizes]
+    got = defects.DefectGraph(nx, ny, nz, *faces).edges()
+    for mask in range(8):
+        qp = tuple(bool(mask >> a & 1) for a in range(3) for _ in range(2))
+        assert got == dr.reference_edges(nx, ny, nz, *faces, q_periodic=qp), qp
+
+
+def test_voxel_pair_graph_last_layer_rule_is_material():
+    """Joining the high-high face pairs of the last voxel layer too (plain
+    all-pairs per voxel) would add an edge the reference never adds."""
+    nx, ny, nz = 4, 4, 4
+    sizes = (nx * (ny - 1) * (nz - 1), (nx - 1) * ny * (nz - 1), (nx - 1) * (ny - 1) * nz)
+    faces = [np.ones(n, dtype=np.uint8) for n in sizes]
+    g = defects.DefectGraph(nx, ny, nz, *faces)
+    ref = dr.reference_edges(nx, ny, nz, *faces)
+    x_hi, y_hi = g.fx(nx - 1, 0, 0), g.nfx + g.fy(nx - 2, 1, 0)     # faces of voxel (nx-2, 0, 0)
+    assert (min(x_hi, y_hi), max(x_hi, y_hi)) not in ref
+    assert g.edges() == ref
+
+
 def test_mesh_and_unstructured_export(tmp_path):
     lines = [np.array([[0.5, 0.5, 0.0], [0.5, 0.5, 1.0], [0.5, 0.5, 2.0]]), np.array([[1.0, 2.5, 0.5]]),
              np.array([[2.5, 1.0, 3.5], [3.5, 1.0, 3.5]])]

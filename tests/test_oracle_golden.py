@@ -97,3 +97,32 @@ def test_order_director_closed_form(oracle_lib):
     order, director = o.order_director()
     np.testing.assert_allclose(order, 2 * S / 3, rtol=1e-10)
     np.testing.assert_allclose(np.abs(np.sum(director * n, axis=1)), 1.0, rtol=1e-10)
+
+
+@pytest.mark.skipif(not os.path.exists("/root/reference/src/lbm_solver.tpp"), reason="reference sources absent")
+def test_oracle_matches_reference_over_the_32cube_1000_step_run(oracle_lib):
+    """The long-trajectory pin: the whole 32^3 FullyPeriodic centred run
+    (1000 steps, seed 12345) through the reference's ActiveNematicSim step
+    order, run now by the reference binary (oracle/_ref/ref_c0), against the
+    oracle. Rounding differences would have 1000 steps to grow; the states
+    must still be equal bit for bit."""
+    import concurrent.futures as cf
+    from lbman3d_amd import abi
+    if not os.path.exists(oracle_lib.ref_binary("c0")):
+        pytest.skip("oracle/_ref not built")
+    dims = (32, 32, 32)
+    q = abi.seed_director_field(12345, *dims, 0.5, 0.01)
+    with cf.ThreadPoolExecutor(1) as ex:
+        fut = ex.submit(oracle_lib.run_ref, "c0", "FullyPeriodic", "full", 0, 1000, dims, q)
+        o = oracle_lib.Oracle(abi.make_config(*dims), threads=4)
+        o.set_state(q=q)
+        o.step(1000)
+        r = fut.result()
+    st = o.state()
+    np.testing.assert_array_equal(st["q"], r["q"])
+    np.testing.assert_array_equal(st["u"], r["u"])
+    np.testing.assert_array_equal(st["rho"], r["rho"])
+    np.testing.assert_array_equal(o.populations(), r["f"])
+    o.set_threads(1)        # the reference binary sums the free energy on one thread
+    assert o.free_energy() == float(r["free_energy"])
+    np.testing.assert_array_equal(np.array(list(o.log_sums().values())), r["log"])

@@ -7,6 +7,7 @@ visit every such cell exactly once and nothing else; here the library's own
 enumeration is compared with a brute-force scan of the grid for every
 boundary preset on thin and ragged grids.
 """
+import ctypes
 import itertools
 
 import numpy as np
@@ -61,3 +62,25 @@ def test_periodic_grid_has_no_shell():
     cfg = abi.make_config(6, 5, 4, abi.FullyPeriodicConfig, precision=abi.FP32_SHIFTED)
     assert shell_region(cfg, 2).shape == (0, 3)
     assert shell_region(cfg, 3).shape == (0, 3)
+
+
+def test_tile_map_stays_inside_the_domain():
+    """Every plane the fused pipelines request (z0 - 2 .. z1 + 1 of a chunk)
+    and every tile column (any distance) maps into [0, n): periodic sides
+    wrap, other sides clamp. One-plane grids were the edge case."""
+    from lbman3d_amd.solver import lib
+    L = lib()
+    out = ctypes.c_int32()
+    for n in range(1, 7):
+        for lo in (0, 1):
+            for hi in (0, 1):
+                for kind, ts in ((0, range(-2, n + 2)), (1, range(-3 * n - 5, 4 * n + 6))):
+                    for t in ts:
+                        assert L.lbm_debug_tile_map(kind, t, n, lo, hi, ctypes.byref(out)) == 0
+                        if t < 0:
+                            want = t % n if lo else 0
+                        elif t >= n:
+                            want = t % n if hi else n - 1
+                        else:
+                            want = t
+                        assert out.value == want, (kind, t, n, lo, hi)

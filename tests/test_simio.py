@@ -115,3 +115,44 @@ def test_run_loop_logs_device_diagnostics(tmp_path, oracle_lib):
     assert float(fields["Kinetic Energy"]) == pytest.approx(ls["kinetic"], rel=1e-10)
     assert float(fields["Max |u|"]) == pytest.approx(ls["umax"], rel=1e-12)
     assert float(fields["Nematic Energy"]) == pytest.approx(o.free_energy(), rel=1e-10)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", [abi.FP64, abi.FP32_SHIFTED])
+def test_device_nonfinite_flag_and_diverged_stop(tmp_path, precision):
+    """A NaN in the simulated state reaches the device reductions: the
+    nonfinite flag is set and the main loop writes the DIVERGED line and
+    stops (src/sim_io.cc:133-165, src/main.cc:17-20). The NaN is data only:
+    IEEE arithmetic carries it and nothing faults. max |u| skips a NaN, as the
+    reference's `if (u2 > umax2)` does (src/sim_io.cc:96)."""
+    from lbman3d_amd import simio
+    from lbman3d_amd.solver import DeviceSolver
+    print(f"case nonfinite precision={precision}", flush=True)
+    dims = (12, 10, 8)
+    V = dims[0] * dims[1] * dims[2]
+    cfg = abi.make_config(*dims, abi.ChannelConfig, precision=precision)
+    q = abi.seed_director_field(5, *dims)
+    rho = np.ones(V)
+    rho[37] = math.nan
+    with DeviceSolver(cfg) as s:
+        s.upload_state(rho=rho, q=q)
+        d = s.diagnostics()
+        assert d["nonfinite"] == 1 and math.isnan(d["mass"])
+        assert math.isfinite(d["umax"])
+        s.upload_state(rho=np.ones(V))        # populations Feq(1, u) again: finite
+        d = s.diagnostics()
+        assert d["nonfinite"] == 0 and d["mass"] == pytest.approx(V, rel=1e-6)
+
+    # A NaN in one cell of Q spreads through the force into the populations;
+    # the Log at step 10 sees a NaN mass.
+    qbad = q.copy()
+    qbad[2, 123] = math.nan
+    path = tmp_path / "lbm.log"
+    done = simio.run(cfg, qbad, 30, 10, "Channel", str(path))
+    assert done == 10
+    text = path.read_text().splitlines()
+    assert text[-2] == "DIVERGED at time step 10 — aborting."
+    assert text[-1] == "LBM program exited."
+    assert sum(1 for ln in text if ln.startswith("Time ")) == 2
+    line10 = [ln for ln in text if ln.startswith("Time 10:")][0]
+    assert "Mass: nan," in line10 or "Mass: -nan," in line10
