@@ -11,10 +11,12 @@ Prints ONE JSON line: MLUPS (whole job), ms per step, the roofline of the
 dominant kernel (algorithmic bytes per launch / mean launch duration measured
 with HIP events on the solver's stream, against the 8 TB/s HBM peak), and a
 CPU baseline: the reference's own OpenMP CPU solver (oracle/_ref/ref_c2,
-compiled from its sources) timed on a few steps of the same workload on this
-host's cores.
+compiled from its sources) timed on 10 warm-up + 30 steps of the same workload
+on the host cores this process may use (CPU model and core counts stated).
+The FP32 result's relative L2 error against the FP64 step (Q, u, rho - rho0)
+at 250 and 1000 steps is reported beside it.
 
-usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline] [--no-accuracy]
 """
 from __future__ import annotations
 
@@ -98,28 +100,98 @@ def roofline(prof, V, schedule_name=None):
                   "kernel_ms": round(ms, 4), "algorithmic_bytes_per_launch": algo * V}
 
 
-def cpu_baseline(q, n, steps=4):
-    """The reference's OpenMP CPU solver on `steps` steps of the same 256^3
-    workload (oracle/_ref/ref_c2: FullyPeriodic, upwind, zeta 0.046, lambda
-    0.7, 16 OpenMP threads), timed by its own step loop."""
+def host_cpu():
+    """CPU model and physical core count of this host (lscpu), and the
+    threads this process may use (affinity, capped by OMP_NUM_THREADS)."""
+    info = {}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, env=dict(os.environ, LC_ALL="C"),
+                             timeout=30).stdout
+        kv = dict(line.split(":", 1) for line in out.splitlines() if ":" in line)
+        kv = {k.strip(): v.strip() for k, v in kv.items()}
+        info["model"] = kv.get("Model name")
+        if kv.get("Core(s) per socket") and kv.get("Socket(s)"):
+            info["physical_cores"] = int(kv["Core(s) per socket"]) * int(kv["Socket(s)"])
+        info["logical_cpus"] = int(kv["CPU(s)"]) if kv.get("CPU(s)") else None
+    except (OSError, ValueError, subprocess.SubprocessError):
+        pass
+    allowed = len(os.sched_getaffinity(0))
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    info["threads"] = min(allowed, int(env)) if env.isdigit() and int(env) > 0 else allowed
+    info["allowed_cpus"] = allowed
+    return info
+
+
+def cpu_baseline(q, n, steps=30, warmup=10):
+    """The reference's OpenMP CPU solver (oracle/_ref/ref_c2: 256^3
+    FullyPeriodic, upwind, zeta 0.046, lambda 0.7, compiled from its sources
+    with the thread count read at run time) on the same workload, under the
+    protocol of src/benchmark.cc:9-50: `warmup` untimed steps, then `steps`
+    steps timed one by one with the QTensor / LBM split. Threads are bound
+    close, one per core (src/mpi/CLAUDE.md "Baseline invocation"), as many as
+    this process may use."""
     exe = os.path.join(ROOT, "oracle", "_ref", "ref_c2")
     if not os.path.exists(exe) or n != 256:
         return None
+    cpu = host_cpu()
     with tempfile.TemporaryDirectory(prefix="lbmcpu_") as td:
         fin = os.path.join(td, "in.bin")
         with open(fin, "wb") as fp:
             np.array([n, n, n, 0, 0], dtype=np.int32).tofile(fp)
             np.ascontiguousarray(q, dtype=np.float64).reshape(-1).tofile(fp)
-        env = dict(os.environ, OMP_PROC_BIND="close", OMP_PLACES="cores")
-        r = subprocess.run([exe, "FullyPeriodic", "full", "0", str(steps), fin, "-"], capture_output=True,
-                           text=True, env=env, timeout=600)
-    m = re.search(r"ref: (\d+) steps of (\d+) cells in ([0-9.]+) ms \(([0-9.]+) MLUPS, (\d+) threads\)", r.stderr)
+        env = dict(os.environ, OMP_NUM_THREADS=str(cpu["threads"]), OMP_PROC_BIND="close", OMP_PLACES="cores")
+        log(f"cpu baseline: reference solver, {cpu['threads']} threads, {warmup} + {steps} steps")
+        r = subprocess.run([exe, "FullyPeriodic", "full", "0", str(steps), fin, "-", str(warmup)],
+                           capture_output=True, text=True, env=env, timeout=600)
+    m = re.search(r"ref: (\d+) steps of (\d+) cells in ([0-9.]+) ms \(([0-9.]+) MLUPS, (\d+) threads; "
+                  r"(\d+) warm-up steps; QTensor ([0-9.]+) ms/step, LBM ([0-9.]+) ms/step\)", r.stderr)
     if r.returncode != 0 or not m:
         log(f"cpu baseline failed: {r.stderr[-400:]}")
         return None
     return {"value": float(m.group(4)), "unit": "MLUPS", "cores": int(m.group(5)), "kind": "reference",
-            "sample": f"{m.group(1)} steps of the 256^3 FP64 workload (reference CPU solver, "
-                      f"{m.group(5)} OpenMP threads, {m.group(3)} ms)"}
+            "sample": f"{m.group(6)} warm-up + {m.group(1)} timed steps of the 256^3 FP64 workload "
+                      f"(reference CPU solver, {m.group(5)} OpenMP threads bound close to cores, "
+                      f"{float(m.group(3)):.0f} ms timed)",
+            "qtensor_ms_per_step": float(m.group(7)), "lbm_ms_per_step": float(m.group(8)),
+            "cpu_model": cpu.get("model"), "host_physical_cores": cpu.get("physical_cores"),
+            "host_logical_cpus": cpu.get("logical_cpus"), "cpus_allotted": cpu["allowed_cpus"]}
+
+
+def accuracy(n, q, checkpoints=(250, 1000)):
+    """Relative L2 error of the FP32 step against the FP64 step (the
+    reference's arithmetic bit for bit) from the same seeded state, in Q, u
+    and rho - rho0, at the given step counts; beside it the chaos floor: the
+    FP64 step started from Qxx perturbed by one ulp in every cell."""
+    from lbman3d_amd import abi
+    from lbman3d_amd.solver import DeviceSolver
+    qp = q.copy()
+    qp[0] = np.nextafter(qp[0], np.inf)
+    sims = {"fp64": DeviceSolver(make_cfg(n, abi.FP64)), "fp32": DeviceSolver(make_cfg(n, abi.FP32_SHIFTED)),
+            "fp64_1ulp": DeviceSolver(make_cfg(n, abi.FP64))}
+    sims["fp64"].upload_state(q=q)
+    sims["fp32"].upload_state(q=q)
+    sims["fp64_1ulp"].upload_state(q=qp)
+    rel = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))
+    out, done = {}, 0
+    try:
+        for cp in checkpoints:
+            for s in sims.values():
+                s.step(cp - done)
+            done = cp
+            ref = sims["fp64"].state()
+            row = {}
+            for k in ("fp32", "fp64_1ulp"):
+                st = sims[k].state()
+                row[k] = {"q": rel(st["q"], ref["q"]), "u": rel(st["u"], ref["u"]),
+                          "rho_minus_rho0": rel(st["rho"] - 1.0, ref["rho"] - 1.0)}
+            row["fp64_umax"] = sims["fp64"].diagnostics()["umax"]
+            out[str(cp)] = row
+            log(f"accuracy at step {cp}: {row}")
+    finally:
+        for s in sims.values():
+            s.close()
+    return {"vs": "FP64 step from the same seeded state (bitwise equal to the reference CPU solver)",
+            "by_step": out}
 
 
 def main():
@@ -130,6 +202,7 @@ def main():
     ap.add_argument("--n", type=int, default=256, help="grid edge")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--schedule", type=int, default=None, help="force an FP32 kernel schedule")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -185,6 +258,9 @@ def main():
                           "fp64_step_hbm_frac": round(STEP_BYTES["fp64"] * V / (ms64 * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                           "fp64_kernels_ms": prof64})
 
+    if not args.no_accuracy:
+        log("accuracy: FP32 and 1-ulp FP64 against FP64")
+        secondary["accuracy"] = accuracy(n, q)
     cpu = None if args.no_cpu_baseline else cpu_baseline(q, n)
     out = {
         "metric": "MLUPS and % peak HBM BW; FP32/batched/closure vs FP64 plain at equal error",

@@ -15,8 +15,20 @@ A network ``ClosureNet`` (lbm_closure_* in the C ABI) maps the coarse state
 after a step (Q, u) to the difference between the filtered fine Q two fine
 steps later and the coarse Q, and is added to Q after every coarse step.
 Training pairs are teacher-forced: each coarse step starts from the filtered
-fine state. The report gives the held-out one-step error with and without
-the closure and a rollout from a held-out fine trajectory.
+fine state. The fine trajectories start from seeded random directors with
+activity zeta drawn uniformly from [0.5, 2] x the params.h default (0.023);
+the held-out trajectories use their own seeds and their own zeta draws. The
+coarse solver of a trajectory runs at that trajectory's zeta; the network
+sees only (Q, u).
+
+The report gives, on the held-out trajectories, the one-step error with and
+without the closure, a 40-coarse-step rollout with and without it, and the
+time-to-solution comparison at equal error: the FP64 plain step run at
+intermediate grids N^3 (N = 64, 80, 96, 112, 128, acoustic scaling for the
+ratio 128/N, started from the Fourier-resampled fine state) is judged by the
+same measure (resampled to 128^3 and box-filtered to 64^3) against the same
+fine trajectory, and its time for the 40 coarse steps (= 80 fine steps) is
+set beside the closure run's.
 
 usage: python -m lbman3d_amd.closure [--iters 1500] [--samples 48]
 """
@@ -123,10 +135,14 @@ class ClosureNet:
 
 # ---- fine / coarse models -------------------------------------------------
 
-def coarse_params(p: abi.Params) -> abi.Params:
-    """Constants of the 2x-coarsened lattice under acoustic scaling."""
-    return abi.Params(DT=p.DT, kDensity=p.kDensity, TAUF=0.5 + (p.TAUF - 0.5) / 2, L=p.L / 4, A=p.A, B=p.B,
-                      C=p.C, LAMBDA=p.LAMBDA, GAMMA=2 * p.GAMMA, ALPHA=p.ALPHA, MU=p.MU)
+ZETA_DEFAULT = 0.023   # params.h ALPHA
+
+
+def coarse_params(p: abi.Params, r: float = 2.0) -> abi.Params:
+    """Constants of a lattice coarsened by the factor r under acoustic
+    scaling (dx and dt both multiplied by r)."""
+    return abi.Params(DT=p.DT, kDensity=p.kDensity, TAUF=0.5 + (p.TAUF - 0.5) / r, L=p.L / (r * r), A=p.A,
+                      B=p.B, C=p.C, LAMBDA=p.LAMBDA, GAMMA=r * p.GAMMA, ALPHA=p.ALPHA, MU=p.MU)
 
 
 def fine_config(n=128, params=None):
@@ -134,9 +150,16 @@ def fine_config(n=128, params=None):
                            precision=abi.FP64)
 
 
-def coarse_config(n=64, params=None):
-    return abi.make_config(n, n, n, abi.FullyPeriodicConfig, coarse_params(params or abi.Params()),
-                           advection=abi.ADV_UPWIND, precision=abi.FP32_SHIFTED)
+def coarse_config(n=64, params=None, r=2.0, precision=abi.FP32_SHIFTED):
+    return abi.make_config(n, n, n, abi.FullyPeriodicConfig, coarse_params(params or abi.Params(), r),
+                           advection=abi.ADV_UPWIND, precision=precision)
+
+
+def trajectories(count, first_seed, draw_seed):
+    """(director seed, zeta) of `count` fine trajectories: zeta uniform in
+    [0.5, 2] x the default."""
+    rng = np.random.default_rng(draw_seed)
+    return [(first_seed + k, float(rng.uniform(0.5, 2.0)) * ZETA_DEFAULT) for k in range(count)]
 
 
 def filter2(a, n):
@@ -145,6 +168,27 @@ def filter2(a, n):
     g = a.reshape(*lead, n // 2, 2, n // 2, 2, n // 2, 2)
     k = len(lead)
     return g.mean(axis=(k + 1, k + 3, k + 5)).reshape(*lead, (n // 2) ** 3)
+
+
+def fourier_resample(a, n_in, n_out):
+    """Periodic band-limited resampling of channel-major fields (..., n_in^3)
+    -> (..., n_out^3): the spectrum is truncated or zero-padded to the modes
+    |k| < min(n_in, n_out) / 2 on each axis (the Nyquist plane is dropped, so
+    the result stays real)."""
+    if n_in == n_out:
+        return a.copy()
+    lead = a.shape[:-1]
+    g = a.reshape(-1, n_in, n_in, n_in)
+    m = min(n_in, n_out) // 2
+    idx_in = np.r_[0:m, n_in - m + 1:n_in]
+    idx_out = np.r_[0:m, n_out - m + 1:n_out]
+    out = np.empty((g.shape[0], n_out, n_out, n_out))
+    for c in range(g.shape[0]):
+        spec = np.fft.fftn(g[c])
+        dst = np.zeros((n_out, n_out, n_out), dtype=complex)
+        dst[np.ix_(idx_out, idx_out, idx_out)] = spec[np.ix_(idx_in, idx_in, idx_in)]
+        out[c] = np.fft.ifftn(dst).real * (n_out / n_in) ** 3
+    return out.reshape(*lead, n_out ** 3)
 
 
 def filtered_state(sim: DeviceSolver, n):
@@ -181,13 +225,15 @@ def generate_pairs(fine: DeviceSolver, coarse: DeviceSolver, n, count, spacing):
 
 
 def fine_targets(fine: DeviceSolver, n, steps, every):
-    """Filtered fine start state and filtered fine Q every `every` coarse steps."""
-    st = filtered_state(fine, n)
+    """Fine state at the start (full and filtered) and the filtered fine Q
+    every `every` coarse steps."""
+    full = fine.state()
+    st = {"rho": filter2(full["rho"], n), "u": filter2(full["u"], n), "q": filter2(full["q"], n)}
     targets = []
     for _ in range(steps // every):
         fine.step(2 * every)
         targets.append(filter2(fine.state()["q"], n))
-    return st, targets
+    return full, st, targets
 
 
 def rollout(coarse: DeviceSolver, net: ClosureNet | None, start, targets, every):
@@ -204,77 +250,148 @@ def rollout(coarse: DeviceSolver, net: ClosureNet | None, start, targets, every)
     return errs
 
 
-def run(n_fine=128, samples=48, heldout=8, spacing=20, warmup=1500, iters=1500, lr=1e-3, roll_steps=40,
-        roll_every=5, seed=11, log=print):
+def plain_rollout(n, n_fine, params, full_start, target, coarse_steps):
+    """FP64 plain run on an n^3 grid (acoustic scaling for n_fine / n) over
+    the span of `coarse_steps` steps of the 2x-coarsened model (= 2 x that
+    many fine steps), from the Fourier-resampled fine state; its Q is
+    resampled to n_fine^3 and box-filtered to the closure's grid to be
+    judged against the same target. Returns (relative L2, ms for the span)."""
+    steps = 2 * coarse_steps * n // n_fine
+    assert steps * n_fine == 2 * coarse_steps * n, "span must be a whole number of steps on this grid"
+    cfg = coarse_config(n, params, r=n_fine / n, precision=abi.FP64)
+    with DeviceSolver(cfg) as s:
+        s.upload_state(rho=fourier_resample(full_start["rho"], n_fine, n),
+                       u=fourier_resample(full_start["u"], n_fine, n),
+                       q=fourier_resample(full_start["q"], n_fine, n))
+        s.step(steps)
+        q = s.state()["q"]
+        err = rel_l2(filter2(fourier_resample(q, n, n_fine), n_fine), target)
+        ms = s.time_steps(steps)
+    return err, ms
+
+
+def equal_error_time(points, err):
+    """Time of the FP64 plain path at a given error, interpolated in
+    (log error, log time) between the measured grids; None if no grid
+    reaches it."""
+    pts = sorted(points, key=lambda p: p["n"])
+    best = None
+    for a, b in zip(pts, pts[1:]):
+        if a["err"] >= err >= b["err"] and a["err"] > b["err"] > 0:
+            t = (np.log(err) - np.log(a["err"])) / (np.log(b["err"]) - np.log(a["err"]))
+            best = float(np.exp(np.log(a["ms"]) + t * (np.log(b["ms"]) - np.log(a["ms"]))))
+            break
+    if best is None and pts and pts[0]["err"] <= err:
+        best = pts[0]["ms"]
+    return best
+
+
+def run(n_fine=128, samples=48, train_traj=4, heldout=8, heldout_traj=2, spacing=20, warmup=1500, iters=1500,
+        lr=1e-3, roll_steps=40, roll_every=5, seed=11, plain_grids=(64, 80, 96, 112, 128), log=print):
     nc = n_fine // 2
     t0 = time.perf_counter()
-    p = abi.Params()
+    p0 = abi.Params()
+    train = trajectories(train_traj, 1000 + seed, seed)
+    held = trajectories(heldout_traj, 2000 + seed, seed + 1)
     report = {"workload": f"closure {nc}^3 coarse vs {n_fine}^3 fine", "fine": f"{n_fine}^3 FP64 upwind",
-              "coarse": f"{nc}^3 FP32 upwind", "coarse_params": coarse_params(p).__dict__}
-    with DeviceSolver(fine_config(n_fine, p)) as fine, DeviceSolver(coarse_config(nc, p)) as coarse:
+              "coarse": f"{nc}^3 FP32 upwind", "coarse_params_at_default_zeta": coarse_params(p0).__dict__,
+              "train_trajectories": [{"seed": sd, "zeta": z} for sd, z in train],
+              "heldout_trajectories": [{"seed": sd, "zeta": z} for sd, z in held]}
+
+    def model(zeta):
+        p = abi.Params(ALPHA=zeta)
+        fine = DeviceSolver(fine_config(n_fine, p))
+        coarse = DeviceSolver(coarse_config(nc, p))
         coarse.set_schedule(1)
-        fine.upload_state(q=seed_director_field(seed, n_fine, n_fine, n_fine))
-        fine.step(warmup)
-        log(f"fine warmup {warmup} steps done, umax {fine.diagnostics()['umax']:.4f}")
-        xs, ys, _ = generate_pairs(fine, coarse, n_fine, samples, spacing)
-        # held-out pairs from a second trajectory
-        fine.upload_state(q=seed_director_field(seed + 1, n_fine, n_fine, n_fine))
-        fine.step(warmup)
-        hx, hy, hbase = generate_pairs(fine, coarse, n_fine, heldout, spacing)
-        t_data = time.perf_counter() - t0
-        log(f"generated {samples} + {heldout} pairs in {t_data:.1f} s")
+        return p, fine, coarse
 
-        X = np.stack(xs)
-        Y = np.stack(ys)
-        in_scale = 1.0 / np.maximum(X.std(axis=(0, 2)), 1e-12)
-        out_scale = 1.0 / np.maximum(Y.std(axis=(0, 2)), 1e-12)
-        with ClosureNet(nc, nc, nc, seed=seed) as net:
-            net.set_normalization(in_scale, out_scale)
-            for x, y in zip(xs, ys):
-                net.add_sample(x, y)
-            t1 = time.perf_counter()
-            curve = []
-            chunk = max(iters // 10, 1)
-            done = 0
-            while done < iters:
-                k = min(chunk, iters - done)
-                curve.append(net.train(k, lr, seed=done))
-                done += k
-                log(f"train {done}/{iters}: mean normalised loss {curve[-1]:.4f}")
-            t_train = time.perf_counter() - t1
+    xs, ys = [], []
+    per = -(-samples // train_traj)
+    for sd, zeta in train:
+        p, fine, coarse = model(zeta)
+        with fine, coarse:
+            fine.upload_state(q=seed_director_field(sd, n_fine, n_fine, n_fine))
+            fine.step(warmup)
+            x, y, _ = generate_pairs(fine, coarse, n_fine, min(per, samples - len(xs)), spacing)
+            xs += x
+            ys += y
+        log(f"train trajectory seed {sd} zeta {zeta:.4f}: {len(xs)} pairs")
+    t_data = time.perf_counter() - t0
 
-            one_no, one_cl = [], []
-            for x, y, (target, qc, _) in zip(hx, hy, hbase):
-                pred = net.predict(x)
-                one_no.append(rel_l2(qc, target))
-                one_cl.append(rel_l2(qc + pred, target))
-            ms_fwd = net.time_forward(20)
-            # rollout on the held-out trajectory (continues after the held-out pairs)
-            start, targets = fine_targets(fine, n_fine, roll_steps, roll_every)
-            err_no = rollout(coarse, None, start, targets, roll_every)
-            err_cl = rollout(coarse, net, start, targets, roll_every)
-            # time to solution per coarse step (= two fine steps)
-            fine_ms = fine.time_steps(100) / 100 * 2
-            coarse_ms = coarse.time_steps(100) / 100
-            reps = 50
-            coarse.synchronize()
-            t2 = time.perf_counter()
-            for _ in range(reps):
-                coarse.step(1)
-                net.apply(coarse, 1.0)
-            coarse.synchronize()
-            coarse_cl_ms = (time.perf_counter() - t2) * 1e3 / reps
+    X = np.stack(xs)
+    Y = np.stack(ys)
+    in_scale = 1.0 / np.maximum(X.std(axis=(0, 2)), 1e-12)
+    out_scale = 1.0 / np.maximum(Y.std(axis=(0, 2)), 1e-12)
+    with ClosureNet(nc, nc, nc, seed=seed) as net:
+        net.set_normalization(in_scale, out_scale)
+        for x, y in zip(xs, ys):
+            net.add_sample(x, y)
+        t1 = time.perf_counter()
+        curve = []
+        chunk = max(iters // 10, 1)
+        done = 0
+        while done < iters:
+            k = min(chunk, iters - done)
+            curve.append(net.train(k, lr, seed=done))
+            done += k
+            log(f"train {done}/{iters}: mean normalised loss {curve[-1]:.4f}")
+        t_train = time.perf_counter() - t1
+        ms_fwd = net.time_forward(20)
 
+        one_no, one_cl, roll_no, roll_cl, plain = [], [], [], [], []
+        timing = {}
+        for sd, zeta in held:
+            p, fine, coarse = model(zeta)
+            with fine, coarse:
+                fine.upload_state(q=seed_director_field(sd, n_fine, n_fine, n_fine))
+                fine.step(warmup)
+                hx, hy, hbase = generate_pairs(fine, coarse, n_fine, max(heldout // heldout_traj, 1), spacing)
+                for x, y, (target, qc, _) in zip(hx, hy, hbase):
+                    pred = net.predict(x)
+                    one_no.append(rel_l2(qc, target))
+                    one_cl.append(rel_l2(qc + pred, target))
+                # rollout on the held-out trajectory (continues after its pairs)
+                full, start, targets = fine_targets(fine, n_fine, roll_steps, roll_every)
+                roll_no.append(rollout(coarse, None, start, targets, roll_every))
+                roll_cl.append(rollout(coarse, net, start, targets, roll_every))
+                pts = []
+                for n in plain_grids:
+                    err, ms = plain_rollout(n, n_fine, p, full, targets[-1], roll_steps)
+                    pts.append({"n": n, "err": err, "ms": ms})
+                plain.append(pts)
+                log(f"held-out seed {sd} zeta {zeta:.4f}: rollout {roll_no[-1][-1]:.3e} -> {roll_cl[-1][-1]:.3e}; "
+                    f"FP64 plain {[(q['n'], round(q['err'], 5)) for q in pts]}")
+                if not timing:
+                    # time to solution of the 40-step span
+                    timing["fine_fp64_ms"] = fine.time_steps(2 * roll_steps)
+                    timing["coarse_fp32_ms"] = coarse.time_steps(roll_steps)
+                    coarse.synchronize()
+                    t2 = time.perf_counter()
+                    for _ in range(roll_steps):
+                        coarse.step(1)
+                        net.apply(coarse, 1.0)
+                    coarse.synchronize()
+                    timing["coarse_with_closure_ms"] = (time.perf_counter() - t2) * 1e3
+
+    roll_no_m = np.mean(np.array(roll_no), axis=0).tolist()
+    roll_cl_m = np.mean(np.array(roll_cl), axis=0).tolist()
+    plain_m = [{"n": n, "err": float(np.mean([pts[k]["err"] for pts in plain])),
+                "ms": float(np.mean([pts[k]["ms"] for pts in plain]))} for k, n in enumerate(plain_grids)]
+    closure_err = roll_cl_m[-1]
+    t_eq = equal_error_time(plain_m, closure_err)
     report.update({
-        "samples": samples, "heldout": heldout, "spacing": spacing, "warmup": warmup, "iters": iters, "lr": lr,
+        "samples": len(xs), "heldout": len(one_no), "spacing": spacing, "warmup": warmup, "iters": iters, "lr": lr,
         "data_s": t_data, "train_s": t_train, "train_ms_per_iter": 1e3 * t_train / iters, "loss_curve": curve,
         "onestep_relL2_no_closure": float(np.mean(one_no)), "onestep_relL2_closure": float(np.mean(one_cl)),
         "onestep_gain": 1.0 - float(np.mean(one_cl)) / float(np.mean(one_no)),
-        "rollout_every": roll_every, "rollout_relL2_no_closure": err_no, "rollout_relL2_closure": err_cl,
+        "rollout_every": roll_every, "rollout_relL2_no_closure": roll_no_m, "rollout_relL2_closure": roll_cl_m,
+        "rollout_per_trajectory_no_closure": roll_no, "rollout_per_trajectory_closure": roll_cl,
         "forward_ms": ms_fwd, "forward_tflops": FLOP_PER_CELL * nc ** 3 / (ms_fwd * 1e-3) / 1e12,
-        "fine_fp64_ms_per_2_steps": fine_ms, "coarse_fp32_ms_per_step": coarse_ms,
-        "coarse_with_closure_wall_ms_per_step": coarse_cl_ms,
-        "speedup_vs_fine_fp64": fine_ms / coarse_cl_ms,
+        "span_coarse_steps": roll_steps, "timing_ms_for_span": timing,
+        "fp64_plain_by_grid": plain_m,
+        "fp64_plain_ms_at_closure_error": t_eq,
+        "speedup_vs_fp64_plain_at_equal_error": (t_eq / timing["coarse_with_closure_ms"]) if t_eq else None,
+        "speedup_vs_fine_fp64": timing["fine_fp64_ms"] / timing["coarse_with_closure_ms"],
     })
     return report
 

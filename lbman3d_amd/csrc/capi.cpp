@@ -161,6 +161,8 @@ BCDesc<T> make_bc(const lbm_config& c) {
     b.n[1] = c.ny;
     b.n[2] = c.nz;
     b.all_periodic_u = 1;
+    // FP32 runs store shifted populations and rho - rho0; FP64 stores rho.
+    b.rho_base = sizeof(T) == 4 ? (T)c.p.density : T(0);
     for (int f = 0; f < 6; ++f) {
         b.qkind[f] = c.walls[f].qbc;
         b.ukind[f] = c.walls[f].ubc;
@@ -279,6 +281,9 @@ int alloc_fields(lbm_solver* s, Fields<T>& f) {
     return LBM_OK;
 }
 
+// What the density array is stored relative to (rho0 for FP32, 0 for FP64).
+double rho_base(const lbm_solver* s) { return s->fp64 ? 0.0 : s->cfg.p.density; }
+
 int fill_value(lbm_solver* s, void* dst, double v) {
     std::vector<char> host(s->V * s->esize());
     if (s->fp64) std::fill((double*)host.data(), (double*)host.data() + s->V, v);
@@ -288,20 +293,21 @@ int fill_value(lbm_solver* s, void* dst, double v) {
     return LBM_OK;
 }
 
-// Copy an FP64 host block of n values into a device field of the handle's type.
-int put_field(lbm_solver* s, void* dst, const double* src, long n) {
+// Copy an FP64 host block of n values into a device field of the handle's
+// type; an FP32 field stores src - base (the density keeps rho - rho0).
+int put_field(lbm_solver* s, void* dst, const double* src, long n, double base = 0.0) {
     if (s->fp64) {
         HIPCHK(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyHostToDevice, s->stream));
         HIPCHK(hipStreamSynchronize(s->stream));
     } else {
         std::vector<float> tmp(n);
-        for (long i = 0; i < n; ++i) tmp[i] = (float)src[i];
+        for (long i = 0; i < n; ++i) tmp[i] = (float)(src[i] - base);
         HIPCHK(hipMemcpyAsync(dst, tmp.data(), n * sizeof(float), hipMemcpyHostToDevice, s->stream));
         HIPCHK(hipStreamSynchronize(s->stream));
     }
     return LBM_OK;
 }
-int get_field(lbm_solver* s, double* dst, const void* src, long n) {
+int get_field(lbm_solver* s, double* dst, const void* src, long n, double base = 0.0) {
     if (s->fp64) {
         HIPCHK(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToHost, s->stream));
         HIPCHK(hipStreamSynchronize(s->stream));
@@ -309,7 +315,7 @@ int get_field(lbm_solver* s, double* dst, const void* src, long n) {
         std::vector<float> tmp(n);
         HIPCHK(hipMemcpyAsync(tmp.data(), src, n * sizeof(float), hipMemcpyDeviceToHost, s->stream));
         HIPCHK(hipStreamSynchronize(s->stream));
-        for (long i = 0; i < n; ++i) dst[i] = (double)tmp[i];
+        for (long i = 0; i < n; ++i) dst[i] = base + (double)tmp[i];
     }
     return LBM_OK;
 }
@@ -559,7 +565,7 @@ int lbm_create(const lbm_config* cfg, lbm_handle* out) {
     s->k32 = make_phys<float>(*cfg);
     // FluidFields constructor state (src/fluid_fields.cc:6-17): rho = kDensity,
     // u = F = 0, populations Feq(rho0, 0).
-    if ((r = fill_value(s, field_ptr(s, 0, 0), cfg->p.density))) return cleanup(r);
+    if ((r = fill_value(s, field_ptr(s, 0, 0), cfg->p.density - rho_base(s)))) return cleanup(r);
     std::vector<double> rho(s->V, cfg->p.density);
     if ((r = lbm_upload_state(s, rho.data(), nullptr, nullptr, nullptr, nullptr))) return cleanup(r);
     *out = s;
@@ -600,7 +606,7 @@ int lbm_upload_state(lbm_handle s, const double* rho, const double* ux, const do
     if (!s) return fail(LBM_ERR_INVALID, "null handle");
     const long V = s->V;
     int r;
-    if (rho && (r = put_field(s, field_ptr(s, 0, 0), rho, V))) return r;
+    if (rho && (r = put_field(s, field_ptr(s, 0, 0), rho, V, rho_base(s)))) return r;
     const double* uu[3] = {ux, uy, uz};
     for (int a = 0; a < 3; ++a)
         if (uu[a] && (r = put_field(s, field_ptr(s, 1, a), uu[a], V))) return r;
@@ -609,7 +615,7 @@ int lbm_upload_state(lbm_handle s, const double* rho, const double* ux, const do
             if ((r = put_field(s, field_ptr(s, 2, m), q + m * V, V))) return r;
     // Populations = Feq(rho, u) of the (now current) device state.
     std::vector<double> hr(V), hu[3];
-    if ((r = get_field(s, hr.data(), field_ptr(s, 0, 0), V))) return r;
+    if ((r = get_field(s, hr.data(), field_ptr(s, 0, 0), V, rho_base(s)))) return r;
     for (int a = 0; a < 3; ++a) {
         hu[a].resize(V);
         if ((r = get_field(s, hu[a].data(), field_ptr(s, 1, a), V))) return r;
@@ -674,7 +680,7 @@ int lbm_download_state(lbm_handle s, double* rho, double* ux, double* uy, double
     if (!s) return fail(LBM_ERR_INVALID, "null handle");
     const long V = s->V;
     int r;
-    if (rho && (r = get_field(s, rho, field_ptr(s, 0, 0), V))) return r;
+    if (rho && (r = get_field(s, rho, field_ptr(s, 0, 0), V, rho_base(s)))) return r;
     double* uu[3] = {ux, uy, uz};
     for (int a = 0; a < 3; ++a)
         if (uu[a] && (r = get_field(s, uu[a], field_ptr(s, 1, a), V))) return r;

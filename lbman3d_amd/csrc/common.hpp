@@ -71,6 +71,8 @@ struct BCDesc {
     T anchor[6][5];      // strong-anchoring Q per face (xx, xy, xz, yy, yz)
     T wall_u[6][3];      // wall velocity per face (MovingWall; zero otherwise)
     int all_periodic_u;  // every velocity face periodic: no wall slots anywhere
+    T rho_base;          // the density array holds rho - rho_base (rho0 for the
+                         // shifted FP32 form, 0 for FP64)
 };
 
 // Physical constants in the arithmetic type of the kernel.
@@ -92,7 +94,7 @@ struct Fields {
     T* F[3];        // body force
     T* u[3];        // velocity u^{n-1} (unfused kernels update it in place)
     T* un[3];       // velocity u^n written by the fused kernel
-    T* rho;         // density
+    T* rho;         // density, stored as rho - BCDesc::rho_base
     T* g;           // populations, source buffer   [15][vs]
     T* gn;          // populations, destination     [15][vs]
     const uint16_t* slots;  // [kClasses][15] pull-slot table

@@ -90,7 +90,7 @@ __device__ __forceinline__ void gather32(const Fields<float>& f, const BCDesc<fl
             out[j] = ldo(g, slot_src(e) * vs4 + c * 4u);
         } else {
             const int face = slot_face(e);
-            if (!have_rho) { rho_prev = ldo(f.rho, c * 4u); have_rho = true; }
+            if (!have_rho) { rho_prev = bc.rho_base + ldo(f.rho, c * 4u); have_rho = true; }
             const float dot = float(cx(j)) * bc.wall_u[face][0] + float(cy(j)) * bc.wall_u[face][1]
                             + float(cz(j)) * bc.wall_u[face][2];
             out[j] = ldo(g, slot_src(e) * vs4 + c * 4u) + 6.0f * rho_prev * float(wgt(j)) * dot;
@@ -583,8 +583,8 @@ __device__ __forceinline__ void fused_body(const Fields<float>& f, const BCDesc<
                 F[1] = fe_prev[1];
                 F[2] = fe_prev[2];
             }
-            float rho, u[3];
-            collide_shifted(k, pop, F, rho, u);
+            float drho, u[3];
+            collide_shifted(k, pop, F, drho, u);
             const unsigned vs4 = (unsigned)f.vs * 4u, c4 = cpb * 4u;
             if (wmask.writes(pb)) {
 #pragma unroll
@@ -594,7 +594,7 @@ __device__ __forceinline__ void fused_body(const Fields<float>& f, const BCDesc<
 #pragma unroll
                 for (int m = 0; m < 5; ++m) sto(f.qn[m], c4, qc[m]);
                 if (aux) {
-                    sto(f.rho, c4, rho);
+                    sto(f.rho, c4, drho);
                     sto(f.F[0], c4, F[0]);
                     sto(f.F[1], c4, F[1]);
                     sto(f.F[2], c4, F[2]);
@@ -841,8 +841,8 @@ fused_step_kernel_pf(Fields<float> f, BCDesc<float> bc, Phys<float> k, int pull,
                 F[1] = fe_prev[1];
                 F[2] = fe_prev[2];
             }
-            float rho, u[3];
-            collide_shifted(k, pop, F, rho, u);
+            float drho, u[3];
+            collide_shifted(k, pop, F, drho, u);
             const unsigned vs4 = (unsigned)f.vs * 4u, c4 = cb * 4u;
 #pragma unroll
             for (int j = 0; j < kQ; ++j) sto_hint<NT>(f.gn, j * vs4 + c4, pop[j]);
@@ -853,7 +853,7 @@ fused_step_kernel_pf(Fields<float> f, BCDesc<float> bc, Phys<float> k, int pull,
             // rho is also an input (moving-wall bounce of the shell cells),
             // so on a domain with walls only the class-0 cells write it.
             if (aux && (!MASKED || cell_mask(bc, 1, gx, gy).writes(pb))) {
-                sto(f.rho, c4, rho);
+                sto(f.rho, c4, drho);
                 sto(f.F[0], c4, F[0]);
                 sto(f.F[1], c4, F[1]);
                 sto(f.F[2], c4, F[2]);
@@ -1059,8 +1059,8 @@ __device__ __forceinline__ void p2_range(const Fields<float>& f, const BCDesc<fl
                 F[1] = fe_prev[1];
                 F[2] = fe_prev[2];
             }
-            float rho, u[3];
-            collide_shifted(k, pop, F, rho, u);
+            float drho, u[3];
+            collide_shifted(k, pop, F, drho, u);
             const unsigned vs4 = (unsigned)f.vs * 4u, c4 = cb * 4u;
 #pragma unroll
             for (int j = 0; j < kQ; ++j) sto_hint<NT>(f.gn, j * vs4 + c4, pop[j]);
@@ -1069,7 +1069,7 @@ __device__ __forceinline__ void p2_range(const Fields<float>& f, const BCDesc<fl
 #pragma unroll
             for (int m = 0; m < 5; ++m) sto_hint<NT>(f.qn[m], c4, qn_prev[m]);
             if (aux && (!MASKED || cell_mask(bc, 1, gx, gy).writes(pb))) {
-                sto(f.rho, c4, rho);
+                sto(f.rho, c4, drho);
                 sto(f.F[0], c4, F[0]);
                 sto(f.F[1], c4, F[1]);
                 sto(f.F[2], c4, F[2]);

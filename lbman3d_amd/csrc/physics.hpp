@@ -248,8 +248,11 @@ __device__ __forceinline__ void collide_plain(const Phys<T>& k, T f[kQ], const T
 //   S = w (omega' (drho - 1.5 rho u^2 + 4.5 rho ue^2) + DT omega_f (-3 u.F + 9 ue eF)),
 // and an odd part A = 3 w (omega' rho ue + DT omega_f eF), so that
 // d*_i = omega d_i + S + A and d*_o = omega d_o + S - A.
+// drho_out is rho - rho0, the form the density array stores next to shifted
+// populations (rho itself, rounded to FP32 near 1, would keep only about
+// 6e-8 of the deviation).
 template <typename T>
-__device__ __forceinline__ void collide_shifted(const Phys<T>& k, T d[kQ], const T F[3], T& rho_out,
+__device__ __forceinline__ void collide_shifted(const Phys<T>& k, T d[kQ], const T F[3], T& drho_out,
                                                 T u_out[3]) {
     // Pair differences p = d_i - d_o carry the momentum; e_i of the first of
     // each pair: 1 (1,0,0), 2 (0,1,0), 5 (0,0,1), 7 (1,1,1), 8 (-1,1,1),
@@ -269,7 +272,7 @@ __device__ __forceinline__ void collide_shifted(const Phys<T>& k, T d[kQ], const
     mx = (mx + hdt * F[0]) * inv;
     my = (my + hdt * F[1]) * inv;
     mz = (mz + hdt * F[2]) * inv;
-    rho_out = rho;
+    drho_out = drho;
     u_out[0] = mx;
     u_out[1] = my;
     u_out[2] = mz;

@@ -67,7 +67,7 @@ __device__ __forceinline__ void gather(const Fields<T>& f, const BCDesc<T>& bc, 
             out[j] = g[slot_src(e) * vs + c];
         } else {
             const int face = slot_face(e);
-            if (!have_rho) { rho_prev = f.rho[c]; have_rho = true; }
+            if (!have_rho) { rho_prev = bc.rho_base + f.rho[c]; have_rho = true; }
             const T dot = T(cx(j)) * bc.wall_u[face][0] + T(cy(j)) * bc.wall_u[face][1]
                         + T(cz(j)) * bc.wall_u[face][2];
             out[j] = g[slot_src(e) * vs + c] + T(6.0) * rho_prev * T(wgt(j)) * dot;
@@ -204,10 +204,10 @@ __device__ __forceinline__ void force_lbm_cell(const Fields<T>& f, const BCDesc<
 
     T pop[kQ];
     gather<T, SHIFTED>(f, bc, p.x, p.y, p.z, c, pull, pop);
-    T rho, u[3];
-    if constexpr (SHIFTED) collide_shifted(k, pop, F, rho, u);
-    else collide_plain(k, pop, F, rho, u);
-    if (flags & FL_STORE_RHO) f.rho[c] = rho;
+    T rho_st, u[3];   // the density in its stored form (rho - BCDesc::rho_base)
+    if constexpr (SHIFTED) collide_shifted(k, pop, F, rho_st, u);
+    else collide_plain(k, pop, F, rho_st, u);
+    if (flags & FL_STORE_RHO) f.rho[c] = rho_st;
     T* const* uo = UN ? f.un : f.u;
     uo[0][c] = u[0];
     uo[1][c] = u[1];
@@ -248,7 +248,7 @@ diag_partial_kernel(Fields<T> f, BCDesc<T> bc, Phys<double> k, long V, double* u
     double acc[kDiag] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     const int nx = bc.n[0], ny = bc.n[1], nz = bc.n[2];
     for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c < V; c += (long)gridDim.x * blockDim.x) {
-        const double r = (double)f.rho[c];
+        const double r = (sizeof(T) == 4 ? k.rho0 : 0.0) + (double)f.rho[c];
         const double ux = (double)f.u[0][c], uy = (double)f.u[1][c], uz = (double)f.u[2][c];
         acc[0] += r;
         acc[1] += r * ux;

@@ -6,7 +6,7 @@
 // that oracle/build_ref.sh puts first on the include path, exactly as the
 // reference's tests shadow it (tests/CMakeLists.txt:3-9).
 //
-// usage: ref_<scenario> <bc> <mode> <force_model> <steps> <in.bin> <out.bin>
+// usage: ref_<scenario> <bc> <mode> <force_model> <steps> <in.bin> <out.bin> [warmup]
 //   bc:    FullyPeriodic | Channel | SlitFreeSlip | SlitNoSlip | PoiseuilleY |
 //          SpecularZ | AnchoredZ | MovingLidZ | SpecularXNoSlipY
 //   mode:  full (Q step, then LBM step) | lbm (LBM only) | q (Q only)
@@ -16,6 +16,9 @@
 //          [rho, ux, uy, uz (4V)], [fx, fy, fz (3V)]
 // out.bin: rho, ux, uy, uz (4V), Q (5V), F (3V), f (15V, i*V + cell),
 //          free energy (1), SimIO::Log sums (8); "-" writes nothing (timing)
+// warmup:  full steps run before the timed loop (timing runs; the protocol
+//          of src/benchmark.cc:9-50: kWarmupSteps = 10 untimed steps, then
+//          the loop timed per step with a QTensor / LBM split)
 
 #include <chrono>
 #include <cstdio>
@@ -87,7 +90,7 @@ std::vector<double> read_block(FILE* fp, size_t n) {
 }
 
 template <typename BC>
-int run(const std::string& mode, int force_model, int steps, const char* in, const char* out) {
+int run(const std::string& mode, int force_model, int steps, const char* in, const char* out, int warmup) {
     FILE* fp = std::fopen(in, "rb");
     if (!fp) { std::perror(in); return 2; }
     int hdr[5];
@@ -123,17 +126,36 @@ int run(const std::string& mode, int force_model, int steps, const char* in, con
     QTensorSolver<BC> qs;
     lbm.Initialize(ff);
 
-    const auto t0 = std::chrono::steady_clock::now();
-    for (int t = 0; t < steps; ++t) {
+    using clock = std::chrono::steady_clock;
+    auto q_step = [&] {
         if (mode == "full" || mode == "q") {
             qs.StepAndSetupBodyForce(qf, ff);
             if (force_model == 0) qs.SetActiveStressAndComputeBodyForce(ff, qf);
         }
+    };
+    auto l_step = [&] {
         if (mode == "full" || mode == "lbm") lbm.LatticeBoltzmannStep(ff);
+    };
+    for (int t = 0; t < warmup; ++t) {
+        q_step();
+        l_step();
     }
-    const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-    std::fprintf(stderr, "ref: %d steps of %zu cells in %.3f ms (%.3f MLUPS, %d threads)\n", steps, V, ms,
-                 steps > 0 ? (double)V * steps / (ms * 1e3) : 0.0, Params::kNumOMPThreads);
+    double ms_q = 0.0, ms_l = 0.0;
+    for (int t = 0; t < steps; ++t) {
+        const auto t0 = clock::now();
+        q_step();
+        const auto t1 = clock::now();
+        l_step();
+        const auto t2 = clock::now();
+        ms_q += std::chrono::duration<double, std::milli>(t1 - t0).count();
+        ms_l += std::chrono::duration<double, std::milli>(t2 - t1).count();
+    }
+    const double ms = ms_q + ms_l;
+    std::fprintf(stderr,
+                 "ref: %d steps of %zu cells in %.3f ms (%.3f MLUPS, %d threads; %d warm-up steps; "
+                 "QTensor %.3f ms/step, LBM %.3f ms/step)\n",
+                 steps, V, ms, steps > 0 ? (double)V * steps / (ms * 1e3) : 0.0, (int)Params::kNumOMPThreads, warmup,
+                 steps > 0 ? ms_q / steps : 0.0, steps > 0 ? ms_l / steps : 0.0);
 
     if (std::string(out) == "-") return 0;   // timing run: no output file
     const double fe = TotalNematicFreeEnergy<BC>(qf);
@@ -171,22 +193,25 @@ int run(const std::string& mode, int force_model, int steps, const char* in, con
 }  // namespace harness
 
 int main(int argc, char** argv) {
-    if (argc != 7) {
-        std::fprintf(stderr, "usage: %s <bc> <full|lbm|q> <force_model> <steps> <in.bin> <out.bin>\n", argv[0]);
+    if (argc != 7 && argc != 8) {
+        std::fprintf(stderr, "usage: %s <bc> <full|lbm|q> <force_model> <steps> <in.bin> <out.bin> [warmup]\n",
+                     argv[0]);
         return 1;
     }
     const std::string bc = argv[1], mode = argv[2];
     const int fm = std::atoi(argv[3]), steps = std::atoi(argv[4]);
+    const int warmup = argc == 8 ? std::atoi(argv[7]) : 0;
+    const char *in = argv[5], *out = argv[6];
     using namespace harness;
-    if (bc == "FullyPeriodic") return run<FullyPeriodicConfig>(mode, fm, steps, argv[5], argv[6]);
-    if (bc == "Channel") return run<ChannelConfig>(mode, fm, steps, argv[5], argv[6]);
-    if (bc == "SlitFreeSlip") return run<SlitConfig>(mode, fm, steps, argv[5], argv[6]);
-    if (bc == "SlitNoSlip") return run<SlitNoSlipConfig>(mode, fm, steps, argv[5], argv[6]);
-    if (bc == "PoiseuilleY") return run<PoiseuilleY>(mode, fm, steps, argv[5], argv[6]);
-    if (bc == "SpecularZ") return run<SpecularZ>(mode, fm, steps, argv[5], argv[6]);
-    if (bc == "AnchoredZ") return run<AnchoredZ>(mode, fm, steps, argv[5], argv[6]);
-    if (bc == "MovingLidZ") return run<MovingLidZ>(mode, fm, steps, argv[5], argv[6]);
-    if (bc == "SpecularXNoSlipY") return run<SpecularXNoSlipY>(mode, fm, steps, argv[5], argv[6]);
+    if (bc == "FullyPeriodic") return run<FullyPeriodicConfig>(mode, fm, steps, in, out, warmup);
+    if (bc == "Channel") return run<ChannelConfig>(mode, fm, steps, in, out, warmup);
+    if (bc == "SlitFreeSlip") return run<SlitConfig>(mode, fm, steps, in, out, warmup);
+    if (bc == "SlitNoSlip") return run<SlitNoSlipConfig>(mode, fm, steps, in, out, warmup);
+    if (bc == "PoiseuilleY") return run<PoiseuilleY>(mode, fm, steps, in, out, warmup);
+    if (bc == "SpecularZ") return run<SpecularZ>(mode, fm, steps, in, out, warmup);
+    if (bc == "AnchoredZ") return run<AnchoredZ>(mode, fm, steps, in, out, warmup);
+    if (bc == "MovingLidZ") return run<MovingLidZ>(mode, fm, steps, in, out, warmup);
+    if (bc == "SpecularXNoSlipY") return run<SpecularXNoSlipY>(mode, fm, steps, in, out, warmup);
     std::fprintf(stderr, "unknown bc %s\n", bc.c_str());
     return 1;
 }

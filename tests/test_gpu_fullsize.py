@@ -43,7 +43,7 @@ def errors(st, ref, rho0=1.0):
 def schedule(name):
     from lbman3d_amd.solver import schedule_name, tile_count
     for s in range(2, 2 + tile_count()):
-        if schedule_name(s) == name:
+        if schedule_name(s) == "fused:" + name:
             return s
     raise KeyError(name)
 
