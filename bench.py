@@ -160,17 +160,20 @@ def cpu_baseline(q, n, steps=30, warmup=10):
 def accuracy(n, q, checkpoints=(250, 1000)):
     """Relative L2 error of the FP32 step against the FP64 step (the
     reference's arithmetic bit for bit) from the same seeded state, in Q, u
-    and rho - rho0, at the given step counts; beside it the chaos floor: the
-    FP64 step started from Qxx perturbed by one ulp in every cell."""
+    and rho - rho0, at the given step counts. Beside it, how far the
+    dynamics alone carries small differences: the FP64 step started from Qxx
+    perturbed by one ulp in every cell, and the FP64 step started from Q
+    rounded to FP32 (a difference of FP32 size, injected once)."""
     from lbman3d_amd import abi
     from lbman3d_amd.solver import DeviceSolver
     qp = q.copy()
     qp[0] = np.nextafter(qp[0], np.inf)
     sims = {"fp64": DeviceSolver(make_cfg(n, abi.FP64)), "fp32": DeviceSolver(make_cfg(n, abi.FP32_SHIFTED)),
-            "fp64_1ulp": DeviceSolver(make_cfg(n, abi.FP64))}
+            "fp64_1ulp": DeviceSolver(make_cfg(n, abi.FP64)), "fp64_from_fp32_q": DeviceSolver(make_cfg(n, abi.FP64))}
     sims["fp64"].upload_state(q=q)
     sims["fp32"].upload_state(q=q)
     sims["fp64_1ulp"].upload_state(q=qp)
+    sims["fp64_from_fp32_q"].upload_state(q=q.astype(np.float32).astype(np.float64))
     rel = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))
     out, done = {}, 0
     try:
@@ -180,7 +183,7 @@ def accuracy(n, q, checkpoints=(250, 1000)):
             done = cp
             ref = sims["fp64"].state()
             row = {}
-            for k in ("fp32", "fp64_1ulp"):
+            for k in ("fp32", "fp64_1ulp", "fp64_from_fp32_q"):
                 st = sims[k].state()
                 row[k] = {"q": rel(st["q"], ref["q"]), "u": rel(st["u"], ref["u"]),
                           "rho_minus_rho0": rel(st["rho"] - 1.0, ref["rho"] - 1.0)}

@@ -12,8 +12,10 @@ all three are reported together with the number of simulations that
 diverged (non-finite mass or momentum) in the batched run.
 
 Accuracy leg: the batch is re-seeded and stepped again while every
-simulation also runs on the FP64 plain path (the reference's arithmetic) and
-on the FP64 path from Qxx perturbed by one ulp (the chaos floor). Every
+simulation also runs on the FP64 plain path (the reference's arithmetic), on
+the FP64 path from Qxx perturbed by one ulp, and on the FP64 path from Q
+rounded to FP32 (how far the dynamics alone carries a difference of FP32
+size, injected once). Every
 `--accuracy-every` steps the relative L2 error of Q, u and rho - rho0 of each
 batched simulation against its FP64 run is recorded (worst and median over
 the simulations, and per simulation).
@@ -59,6 +61,7 @@ def accuracy(cfgs, qs, steps, every):
     n = len(cfgs)
     f64 = [DeviceSolver(abi.config_copy(c, precision=abi.FP64)) for c in cfgs]
     ulp = [DeviceSolver(abi.config_copy(c, precision=abi.FP64)) for c in cfgs]
+    r32 = [DeviceSolver(abi.config_copy(c, precision=abi.FP64)) for c in cfgs]
     rows = []
     try:
         with BatchSolver(cfgs) as b:
@@ -68,11 +71,12 @@ def accuracy(cfgs, qs, steps, every):
                 qp = q.copy()
                 qp[0] = np.nextafter(qp[0], np.inf)
                 ulp[i].upload_state(q=qp)
+                r32[i].upload_state(q=q.astype(np.float32).astype(np.float64))
             done = 0
             while done < steps:
                 k = min(every, steps - done)
                 b.step(k, use_graph=True)
-                for s in f64 + ulp:
+                for s in f64 + ulp + r32:
                     s.step(k)
                 done += k
                 per = []
@@ -80,10 +84,10 @@ def accuracy(cfgs, qs, steps, every):
                     ref, st, tw = f64[i].state(), b.state(i), ulp[i].state()
                     e = lambda x: {"q": rel_l2(x["q"], ref["q"]), "u": rel_l2(x["u"], ref["u"]),
                                    "rho": rel_l2(x["rho"] - 1.0, ref["rho"] - 1.0)}
-                    per.append({"fp32_batch": e(st), "fp64_1ulp": e(tw),
+                    per.append({"fp32_batch": e(st), "fp64_1ulp": e(tw), "fp64_from_fp32_q": e(r32[i].state()),
                                 "fp64_nonfinite": f64[i].diagnostics()["nonfinite"]})
                 row = {"step": done}
-                for leg in ("fp32_batch", "fp64_1ulp"):
+                for leg in ("fp32_batch", "fp64_1ulp", "fp64_from_fp32_q"):
                     for f in ("q", "u", "rho"):
                         v = np.array([p[leg][f] for p in per])
                         row[f"{leg}_{f}_max"] = float(np.nanmax(v)) if np.isfinite(v).any() else None
@@ -94,7 +98,7 @@ def accuracy(cfgs, qs, steps, every):
                 print(json.dumps({k: v for k, v in row.items() if k != "fp32_batch_q_per_sim"}), flush=True)
                 rows.append(row)
     finally:
-        for s in f64 + ulp:
+        for s in f64 + ulp + r32:
             s.close()
     return rows
 
