@@ -174,6 +174,13 @@ int lbm_diagnostics(lbm_handle h, lbm_diag* out);
  * (QtensorToOrderDirector, src/analysis_fields.cc:59-93); director is
  * [V][3] interleaved. Computed on the device in FP64. */
 int lbm_order_director(lbm_handle h, double* order, double* director);
+/* lbm_order_director without any device allocation. This entry point has no
+ * counterpart in the reference: it is for callers that cannot allocate device
+ * memory (tight serving loops, harnesses that forbid hipMalloc). The fields
+ * are staged in a fixed 16 MiB device array per GPU, so grids are limited to
+ * 524,288 cells (larger ones return LBM_ERR_INVALID); calls that overlap on
+ * one GPU race on that array. Results equal lbm_order_director's. */
+int lbm_order_director_noalloc(lbm_handle h, double* order, double* director);
 
 /* Plaquettes pierced by a disclination line (DefectFinder::ComputeWindingNumbers,
  * src/analysis/defect_finder.tpp:12-162): 1 where the director of

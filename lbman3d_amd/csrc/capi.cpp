@@ -13,6 +13,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -809,6 +810,58 @@ int lbm_order_director(lbm_handle s, double* order, double* director) {
         r = fail(LBM_ERR_HIP, "order/director failed");
     (void)hipFree(tmp);
     return r;
+}
+
+// ---- order/director without device allocation --------------------------------
+// lbm_order_director_noalloc stages the order and director fields in one
+// file-scope device array instead of a per-call hipMalloc. The array's name is
+// a host-side stand-in: host code uses only the address hipGetSymbolAddress
+// returns for the current device (looked up once per device, on that device's
+// first call), never the name itself. Every call on one GPU shares that GPU's
+// array and overwrites all of it that it reads back (nothing needs clearing);
+// calls that overlap on one GPU (two host threads, or two streams) race.
+namespace {
+constexpr long kNoAllocMaxCells = 1L << 19;   // 524,288 cells; 16 MiB of staging
+__device__ __align__(256) double scratch[4 * kNoAllocMaxCells];
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+int scratch_for_current_device(double** out) {
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return fail(LBM_ERR_UNSUPPORTED, "device index beyond the staging table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(LBM_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_ptr[dev] = (double*)p;
+    }
+    *out = g_scratch_ptr[dev];
+    return LBM_OK;
+}
+}  // namespace
+
+int lbm_order_director_noalloc(lbm_handle s, double* order, double* director) {
+    if (!s || !order || !director) return fail(LBM_ERR_INVALID, "null argument");
+    if (s->V > kNoAllocMaxCells)
+        return fail(LBM_ERR_INVALID, "grid of " + std::to_string(s->V) + " cells exceeds the " +
+                                         std::to_string(kNoAllocMaxCells) +
+                                         "-cell staging limit of lbm_order_director_noalloc");
+    double* base = nullptr;
+    if (int r = scratch_for_current_device(&base)) return r;
+    double* o = base;
+    double* d = base + s->V;
+    if (s->fp64) launch_order_director(s->d64, s->V, o, d, s->stream);
+    else launch_order_director(s->d32, s->V, o, d, s->stream);
+    if (hipGetLastError() != hipSuccess ||
+        hipMemcpyAsync(order, o, s->V * sizeof(double), hipMemcpyDeviceToHost, s->stream) != hipSuccess ||
+        hipMemcpyAsync(director, d, 3 * s->V * sizeof(double), hipMemcpyDeviceToHost, s->stream) != hipSuccess ||
+        hipStreamSynchronize(s->stream) != hipSuccess)
+        return fail(LBM_ERR_HIP, "order/director failed");
+    return LBM_OK;
 }
 
 int lbm_time_steps(lbm_handle s, int32_t n, double* ms) {

@@ -41,6 +41,7 @@ SIGNATURES = {
     "lbm_synchronize": (C.c_int, [C.c_void_p]),
     "lbm_diagnostics": (C.c_int, [C.c_void_p, C.POINTER(LbmDiag)]),
     "lbm_order_director": (C.c_int, [C.c_void_p, _D, _D]),
+    "lbm_order_director_noalloc": (C.c_int, [C.c_void_p, _D, _D]),
     "lbm_time_steps": (C.c_int, [C.c_void_p, C.c_int32, _D]),
     "lbm_profile_kernels": (C.c_int, [C.c_void_p, C.c_int32, _D, C.c_int32, C.c_char_p, C.c_size_t,
                                       C.POINTER(C.c_int32)]),
@@ -237,9 +238,12 @@ class DeviceSolver:
         check(lib().lbm_diagnostics(self.h, C.byref(d)))
         return d.as_dict()
 
-    def order_director(self):
+    def order_director(self, noalloc=False):
+        """Order parameter and director per cell; noalloc uses the entry
+        point that stages in a fixed device array (grids up to 2^19 cells)."""
         o, d = np.empty(self.V), np.empty(3 * self.V)
-        check(lib().lbm_order_director(self.h, _p(o), _p(d)))
+        fn = lib().lbm_order_director_noalloc if noalloc else lib().lbm_order_director
+        check(fn(self.h, _p(o), _p(d)))
         return o, d.reshape(self.V, 3)
 
     def time_steps(self, n) -> float:
