@@ -311,6 +311,10 @@ int lbm_closure_predict(lbm_closure_handle c, const double* x, double* y);
  * pending work (later closure calls wait for it); calls on one closure must
  * not overlap from several threads. */
 int lbm_closure_apply(lbm_closure_handle c, lbm_handle sim, double alpha);
+/* n coarse steps of sim, each followed by the closure correction (alpha as
+ * in lbm_closure_apply), launched back to back on the simulation's stream;
+ * ms (optional) receives their device time from HIP events. */
+int lbm_closure_run(lbm_closure_handle c, lbm_handle sim, int32_t n, double alpha, double* ms);
 /* Mean time of one forward pass over the whole grid (ms), n passes. */
 int lbm_closure_time_forward(lbm_closure_handle c, int32_t n, double* ms);
 /* D = A (16x32, row-major) * B (32x16) through the BF16 matrix-core

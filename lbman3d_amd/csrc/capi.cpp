@@ -749,12 +749,16 @@ int lbm_lbm_step(lbm_handle s) {
 // depends on the grid (small grids give the fused z-marching kernel too few
 // tiles to fill 256 CUs). Tuning runs on a snapshot and leaves the state as
 // it was; if it cannot run, the default fused tile is used.
-static void ensure_tuned(lbm_handle s) {
+}  // extern "C"
+namespace lbm {
+void ensure_tuned(lbm_solver* s) {
     if (s->fp64 || s->schedule != 0 || s->tuned != 0) return;
     const std::string saved_error = g_last_error;
     if (lbm_autotune(s, 3, nullptr, 0, nullptr) != LBM_OK) s->tuned = s->fused_ok ? 2 : 1;
     g_last_error = saved_error;
 }
+}  // namespace lbm
+extern "C" {
 
 int lbm_step(lbm_handle s, int32_t n) {
     if (!s) return fail(LBM_ERR_INVALID, "null handle");

@@ -44,4 +44,6 @@ struct lbm_solver {
 namespace lbm {
 // One full step of a handle (capi.cpp); `last` materialises rho and F.
 int run_step(lbm_solver* s, bool last, KernelRecorder* rec);
+// Tunes the automatic FP32 schedule on first use (capi.cpp).
+void ensure_tuned(lbm_solver* s);
 }  // namespace lbm

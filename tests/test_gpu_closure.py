@@ -157,3 +157,29 @@ def test_rejects_bad_shapes():
         ClosureNet(20, 8, 8)
     with pytest.raises(LbmError):
         ClosureNet(32, 8, 8, hidden=16)
+
+
+def test_run_equals_step_and_apply_loop():
+    """lbm_closure_run(n) gives the bits of n x (lbm_step(1); lbm_closure_apply)."""
+    from lbman3d_amd import abi
+    from lbman3d_amd.solver import DeviceSolver
+    print("case closure run loop", flush=True)
+    nx, ny, nz = GRID
+    cfg = abi.make_config(nx, ny, nz, precision=abi.FP32_SHIFTED)
+    q0 = abi.seed_director_field(6, nx, ny, nz)
+    out = {}
+    for use_run in (False, True):
+        with DeviceSolver(cfg) as s, net(seed=3) as n:
+            s.set_schedule(2)
+            s.upload_state(q=q0)
+            n.set_normalization(np.full(8, 2.0), np.full(5, 50.0))
+            if use_run:
+                ms = n.run(s, 4, 0.5)
+                assert ms > 0
+            else:
+                for _ in range(4):
+                    s.step(1)
+                    n.apply(s, 0.5)
+            out[use_run] = s.state()
+    np.testing.assert_array_equal(out[True]["q"], out[False]["q"])
+    np.testing.assert_array_equal(out[True]["u"], out[False]["u"])
