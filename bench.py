@@ -40,6 +40,7 @@ HBM_PEAK_GBS = 8000.0
 # the two-kernel split, the compulsory traffic of each kernel on its own.
 ALGO_BYTES = {
     "fused_step_f32": 46 * 4,
+    "fused_step_f64": 46 * 8,
     "qstep_f32": (8 + 16) * 4,          # Q, u in; Q', Sigma, Tau, F out
     "force_lbm_f32": (5 + 8 + 3 + 15 + 15 + 3) * 4,
     "qstep_f64": (8 + 16) * 8,
@@ -255,11 +256,26 @@ def main():
         log("timing FP64 plain step")
         k64 = max(10, args.steps // 5)
         s64, ev64, _, prof64, _, _ = time_solver(make_cfg(n, abi.FP64), q, k64, 5)
+        name64 = s64.schedule_name(0)
+        # every FP64 schedule: the two-kernel split and the fused tiles
+        by_sched = {}
+        sched = 1
+        while True:
+            try:
+                nm = s64.schedule_name(sched)
+            except Exception:
+                break
+            s64.set_schedule(sched)
+            s64.step(3)
+            by_sched[nm] = round(s64.time_steps(k64) / k64, 4)
+            sched += 1
         s64.close()
         ms64 = ev64 / k64
+        k64name, roof64 = roofline(prof64, V)
         secondary.update({"fp64_ms_per_step": round(ms64, 4), "fp64_mlups": round(V / (ms64 * 1e-3) / 1e6, 1),
+                          "fp64_schedule": name64,
                           "fp64_step_hbm_frac": round(STEP_BYTES["fp64"] * V / (ms64 * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-                          "fp64_kernels_ms": prof64})
+                          "fp64_kernels_ms": prof64, "fp64_roofline": roof64, "fp64_schedules_ms_per_step": by_sched})
 
     if not args.no_accuracy:
         log("accuracy: FP32 and 1-ulp FP64 against FP64")

@@ -32,6 +32,9 @@ using namespace lbm;
 // ---------------------------------------------------------------------------
 namespace lbm {
 thread_local std::string g_last_error;
+// FP64 fused tile used by the automatic schedule on fully periodic domains.
+constexpr int kDefaultTile64 = 1;   // fp64:32x4x128, the fastest at 256^3
+
 int fail(int code, const std::string& msg) {
     g_last_error = msg;
     return code;
@@ -405,7 +408,11 @@ int run_lbm(lbm_solver* s, KernelRecorder* rec) {
 
 // Fused tile index for the next step, or -1 for the two-kernel split.
 int active_tile(const lbm_solver* s) {
-    if (s->fp64 || !s->fused_ok) return -1;
+    if (s->fp64) {
+        if (!s->fused64_ok || s->schedule == 1) return -1;
+        return s->schedule >= 2 ? s->schedule - 2 : kDefaultTile64;
+    }
+    if (!s->fused_ok) return -1;
     int sched = s->schedule ? s->schedule : (s->tuned ? s->tuned : 2);
     return sched >= 2 ? sched - 2 : -1;
 }
@@ -414,7 +421,14 @@ int active_tile(const lbm_solver* s) {
 // auxiliary outputs (rho, F) are materialised for downloads.
 int run_step(lbm_solver* s, bool last, KernelRecorder* rec) {
     hipStream_t st = s->stream;
-    if (s->fp64) {
+    if (s->fp64 && active_tile(s) >= 0) {
+        if (rec) rec->begin("fused_step_f64");
+        launch_fused_step_f64(s->d64, s->bc64, s->k64, active_tile(s), s->pulled, (last || s->moving) ? 1 : 0, st);
+        if (rec) rec->end();
+        swap_q(s->d64);
+        swap_g(s->d64);
+        swap_u(s->d64);
+    } else if (s->fp64) {
         if (rec) rec->begin("qstep_f64");
         launch_qstep(s->d64, s->bc64, s->k64, s->V, st);
         if (rec) rec->end();
@@ -533,6 +547,13 @@ int lbm_create(const lbm_config* cfg, lbm_handle* out) {
     s->moving = has_moving_wall(*cfg);
     // The fused kernel addresses populations with 32-bit byte offsets.
     s->fused_ok = !s->fp64 && fused_tile_count() > 0 && (double)kQ * s->vs * sizeof(float) < 4294967296.0;
+    {
+        bool periodic = true;
+        for (int face = 0; face < 6; ++face)
+            periodic = periodic && cfg->walls[face].qbc == LBM_Q_PERIODIC && cfg->walls[face].ubc == LBM_U_PERIODIC;
+        s->fused64_ok = s->fp64 && periodic && fused64_tile_count() > 0 &&
+                        (double)kQ * s->vs * sizeof(double) < 4294967296.0;
+    }
     auto cleanup = [&](int r) {
         lbm_destroy(s);
         return r;
@@ -913,7 +934,7 @@ int lbm_profile_kernels(lbm_handle s, int32_t n, double* ms_out, int32_t max, ch
 
 int lbm_set_schedule(lbm_handle s, int32_t schedule) {
     if (!s) return fail(LBM_ERR_INVALID, "null handle");
-    if (schedule < 0 || schedule > 1 + fused_tile_count())
+    if (schedule < 0 || schedule > 1 + (s->fp64 ? fused64_tile_count() : fused_tile_count()))
         return fail(LBM_ERR_INVALID, "schedule out of range");
     s->schedule = schedule;
     return LBM_OK;
@@ -930,6 +951,19 @@ int lbm_tile_name(int32_t schedule, char* buf, size_t len) {
     if (schedule == 0) n = "auto";
     else if (schedule == 1) n = "two-kernel";
     else if (schedule - 2 < fused_tile_count()) n = std::string("fused:") + fused_tile_name(schedule - 2);
+    else return fail(LBM_ERR_INVALID, "schedule out of range");
+    std::strncpy(buf, n.c_str(), len - 1);
+    buf[len - 1] = 0;
+    return LBM_OK;
+}
+
+int lbm_schedule_name(lbm_handle s, int32_t schedule, char* buf, size_t len) {
+    if (!s || !buf || !len) return fail(LBM_ERR_INVALID, "null argument");
+    if (!s->fp64) return lbm_tile_name(schedule, buf, len);
+    std::string n;
+    if (schedule == 0) n = s->fused64_ok ? std::string("auto:") + fused64_tile_name(kDefaultTile64) : "auto:two-kernel";
+    else if (schedule == 1) n = "two-kernel";
+    else if (schedule - 2 < fused64_tile_count()) n = std::string("fused:") + fused64_tile_name(schedule - 2);
     else return fail(LBM_ERR_INVALID, "schedule out of range");
     std::strncpy(buf, n.c_str(), len - 1);
     buf[len - 1] = 0;

@@ -39,6 +39,12 @@ __device__ __forceinline__ float ldo(const float* base, unsigned boff) {
 __device__ __forceinline__ void sto(float* base, unsigned boff, float v) {
     *reinterpret_cast<float*>(reinterpret_cast<char*>(base) + boff) = v;
 }
+__device__ __forceinline__ double ldo(const double* base, unsigned boff) {
+    return *reinterpret_cast<const double*>(reinterpret_cast<const char*>(base) + boff);
+}
+__device__ __forceinline__ void sto(double* base, unsigned boff, double v) {
+    *reinterpret_cast<double*>(reinterpret_cast<char*>(base) + boff) = v;
+}
 // Store with an optional non-temporal hint (streaming output read back only
 // next step, so it need not displace the halo rows held in L2).
 template <bool NT>
@@ -48,16 +54,18 @@ __device__ __forceinline__ void sto_hint(float* base, unsigned boff, float v) {
     else *p = v;
 }
 
-// Pull gather with 32-bit offsets (fused kernel; requires 15 * vs * 4 < 2^32).
-template <bool PERIODIC>
-__device__ __forceinline__ void gather32(const Fields<float>& f, const BCDesc<float>& bc, int x, int y, int z,
-                                         unsigned c, int pull, float out[kQ]) {
+// Pull gather with 32-bit byte offsets (fused kernels; requires
+// 15 * vs * sizeof(T) < 2^32).
+template <bool PERIODIC, typename T>
+__device__ __forceinline__ void gather32(const Fields<T>& f, const BCDesc<T>& bc, int x, int y, int z,
+                                         unsigned c, int pull, T out[kQ]) {
+    constexpr unsigned E = sizeof(T);
     const int nx = bc.n[0], ny = bc.n[1], nz = bc.n[2];
-    const unsigned vs4 = (unsigned)f.vs * 4u;
-    const float* g = f.g;
+    const unsigned vs4 = (unsigned)f.vs * E;
+    const T* g = f.g;
     if (!pull) {
 #pragma unroll
-        for (int j = 0; j < kQ; ++j) out[j] = ldo(g, j * vs4 + c * 4u);
+        for (int j = 0; j < kQ; ++j) out[j] = ldo(g, j * vs4 + c * E);
         return;
     }
     const int cls = node_class(x, y, z, nx, ny, nz);
@@ -69,12 +77,12 @@ __device__ __forceinline__ void gather32(const Fields<float>& f, const BCDesc<fl
 #pragma unroll
         for (int j = 0; j < kQ; ++j) {
             const unsigned src = ((unsigned)zs[cz(j) + 1] * ny + ys[cy(j) + 1]) * nx + xs[cx(j) + 1];
-            out[j] = ldo(g, j * vs4 + src * 4u);
+            out[j] = ldo(g, j * vs4 + src * E);
         }
         return;
     }
     const uint16_t* row = f.slots + cls * kQ;
-    float rho_prev = 0.0f;
+    T rho_prev = T(0);
     bool have_rho = false;
 #pragma unroll
     for (int j = 0; j < kQ; ++j) {
@@ -85,15 +93,15 @@ __device__ __forceinline__ void gather32(const Fields<float>& f, const BCDesc<fl
             xs = xs < 0 ? xs + nx : (xs >= nx ? xs - nx : xs);
             ys = ys < 0 ? ys + ny : (ys >= ny ? ys - ny : ys);
             zs = zs < 0 ? zs + nz : (zs >= nz ? zs - nz : zs);
-            out[j] = ldo(g, j * vs4 + (((unsigned)zs * ny + ys) * nx + xs) * 4u);
+            out[j] = ldo(g, j * vs4 + (((unsigned)zs * ny + ys) * nx + xs) * E);
         } else if (kind == SLOT_MIRROR) {
-            out[j] = ldo(g, slot_src(e) * vs4 + c * 4u);
+            out[j] = ldo(g, slot_src(e) * vs4 + c * E);
         } else {
             const int face = slot_face(e);
-            if (!have_rho) { rho_prev = bc.rho_base + ldo(f.rho, c * 4u); have_rho = true; }
-            const float dot = float(cx(j)) * bc.wall_u[face][0] + float(cy(j)) * bc.wall_u[face][1]
-                            + float(cz(j)) * bc.wall_u[face][2];
-            out[j] = ldo(g, slot_src(e) * vs4 + c * 4u) + 6.0f * rho_prev * float(wgt(j)) * dot;
+            if (!have_rho) { rho_prev = bc.rho_base + ldo(f.rho, c * E); have_rho = true; }
+            const T dot = T(cx(j)) * bc.wall_u[face][0] + T(cy(j)) * bc.wall_u[face][1]
+                        + T(cz(j)) * bc.wall_u[face][2];
+            out[j] = ldo(g, slot_src(e) * vs4 + c * E) + T(6.0) * rho_prev * T(wgt(j)) * dot;
         }
     }
 }
@@ -122,17 +130,20 @@ struct AxisTags {
 };
 
 // Q-update (stage A) results kept per plane.
-struct StageA {
-    float qn[5], sg[5], ta[3];
+template <typename T>
+struct StageAT {
+    T qn[5], sg[5], ta[3];
 };
+using StageA = StageAT<float>;
 
 constexpr int kNumIn = 8;    // Q (5) + u (3)
 constexpr int kNumA = 13;    // Q^{n+1} (5) + Sigma (5) + Tau (3)
 
-template <int TX, int TY>
-__device__ __forceinline__ void load_column(const Fields<float>& f, int qcol, int ucol_xy, unsigned qplane,
-                                            unsigned uplane, float v[kNumIn]) {
-    const unsigned qo = (qplane + (unsigned)qcol) * 4u, uo = (uplane + (unsigned)ucol_xy) * 4u;
+template <int TX, int TY, typename T>
+__device__ __forceinline__ void load_column(const Fields<T>& f, int qcol, int ucol_xy, unsigned qplane,
+                                            unsigned uplane, T v[kNumIn]) {
+    constexpr unsigned E = sizeof(T);
+    const unsigned qo = (qplane + (unsigned)qcol) * E, uo = (uplane + (unsigned)ucol_xy) * E;
 #pragma unroll
     for (int m = 0; m < 5; ++m) v[m] = ldo(f.q[m], qo);
 #pragma unroll
@@ -141,29 +152,29 @@ __device__ __forceinline__ void load_column(const Fields<float>& f, int qcol, in
 
 // Stage A at one column: Beris-Edwards update from the input plane in LDS
 // (xy neighbours) and the column's own z neighbours.
-template <int TX, int TY, bool NOGHOST>
-__device__ __forceinline__ void stage_a(const BCDesc<float>& bc, const Phys<float>& k,
-                                        const float (*sIn)[TileGeom<TX, TY>::NI], int ii, int jj,
-                                        const float zm[kNumIn], const float c[kNumIn], const float zp[kNumIn],
-                                        const int qpos[3], const int upos[3], StageA& out, float fe[3]) {
+template <int TX, int TY, bool NOGHOST, typename T>
+__device__ __forceinline__ void stage_a(const BCDesc<T>& bc, const Phys<T>& k,
+                                        const T (*sIn)[TileGeom<TX, TY>::NI], int ii, int jj,
+                                        const T zm[kNumIn], const T c[kNumIn], const T zp[kNumIn],
+                                        const int qpos[3], const int upos[3], StageAT<T>& out, T fe[3]) {
     using G = TileGeom<TX, TY>;
     const int cc = jj * G::IX + ii;
     const int xm = cc - 1, xp = cc + 1, ym = cc - G::IX, yp = cc + G::IX;
-    float Q[5];
-    Deriv<float> dq[5];
+    T Q[5];
+    Deriv<T> dq[5];
 #pragma unroll
     for (int m = 0; m < 5; ++m) {
         Q[m] = c[m];
-        dq[m] = q_deriv<float, NOGHOST>(bc, m, qpos, c[m], sIn[m][xm], sIn[m][xp], sIn[m][ym], sIn[m][yp], zm[m], zp[m]);
+        dq[m] = q_deriv<T, NOGHOST>(bc, m, qpos, c[m], sIn[m][xm], sIn[m][xp], sIn[m][ym], sIn[m][yp], zm[m], zp[m]);
     }
-    float u[3], g[3][3];
+    T u[3], g[3][3];
 #pragma unroll
     for (int A = 0; A < 3; ++A) {
-        const float v0 = c[5 + A];
+        const T v0 = c[5 + A];
         u[A] = v0;
-        g[A][0] = u_grad<float, NOGHOST>(bc, 0, A, upos[0], sIn[5 + A][xm], v0, sIn[5 + A][xp]);
-        g[A][1] = u_grad<float, NOGHOST>(bc, 1, A, upos[1], sIn[5 + A][ym], v0, sIn[5 + A][yp]);
-        g[A][2] = u_grad<float, NOGHOST>(bc, 2, A, upos[2], zm[5 + A], v0, zp[5 + A]);
+        g[A][0] = u_grad<T, NOGHOST>(bc, 0, A, upos[0], sIn[5 + A][xm], v0, sIn[5 + A][xp]);
+        g[A][1] = u_grad<T, NOGHOST>(bc, 1, A, upos[1], sIn[5 + A][ym], v0, sIn[5 + A][yp]);
+        g[A][2] = u_grad<T, NOGHOST>(bc, 2, A, upos[2], zm[5 + A], v0, zp[5 + A]);
     }
     q_pointwise(k, Q, u, dq, g, out.qn, out.sg, out.ta, fe);
 }
@@ -308,7 +319,8 @@ __device__ __forceinline__ int dz_slot(int m) {
         default: return -1;
     }
 }
-__device__ __forceinline__ float stage_value(const StageA& s, int m) {
+template <typename T>
+__device__ __forceinline__ T stage_value(const StageAT<T>& s, int m) {
     return m < 5 ? s.qn[m] : (m < 10 ? s.sg[m - 5] : s.ta[m - 10]);
 }
 
@@ -335,7 +347,8 @@ struct CellMask {
         return (mode == 0) | ((mode == 1) & (cls == 0)) | ((mode >= 2) & (cls == mode - 1));
     }
 };
-__device__ __forceinline__ CellMask cell_mask(const BCDesc<float>& bc, int mode, int x, int y) {
+template <typename T>
+__device__ __forceinline__ CellMask cell_mask(const BCDesc<T>& bc, int mode, int x, int y) {
     int open[6];
 #pragma unroll
     for (int face = 0; face < 6; ++face)
@@ -355,12 +368,13 @@ __device__ __forceinline__ CellMask cell_mask(const BCDesc<float>& bc, int mode,
 // column tile (tx, ty), writing the cells that `mode` selects. LDS buffers
 // come from the calling kernel so that a kernel instantiating several
 // variants does not allocate LDS per variant.
-template <int TX, int TY, int ZC, bool PERIODIC>
-__device__ __forceinline__ void fused_body(const Fields<float>& f, const BCDesc<float>& bc, const Phys<float>& k,
+template <int TX, int TY, int ZC, bool PERIODIC, typename T>
+__device__ __forceinline__ void fused_body(const Fields<T>& f, const BCDesc<T>& bc, const Phys<T>& k,
                                            int pull, int aux, int tx, int ty, int tz,
-                                           float (*sIn)[TileGeom<TX, TY>::NI],
-                                           float (*sA)[kNumA][TileGeom<TX, TY>::NA],
-                                           float (*sRing)[kNumIn][TileGeom<TX, TY>::NR], int mode = 0) {
+                                           T (*sIn)[TileGeom<TX, TY>::NI],
+                                           T (*sA)[kNumA][TileGeom<TX, TY>::NA],
+                                           T (*sRing)[kNumIn][TileGeom<TX, TY>::NR], int mode = 0) {
+    constexpr unsigned E = sizeof(T);
     using G = TileGeom<TX, TY>;
     const int nx = bc.n[0], ny = bc.n[1], nz = bc.n[2];
     const int x0 = tx * TX, y0 = ty * TY;
@@ -413,11 +427,11 @@ __device__ __forceinline__ void fused_body(const Fields<float>& f, const BCDesc<
     auto uplane = [&](int z) { return (unsigned)map_coord(z, nz, ax[2].ulo, ax[2].uhi) * plane; };
 
     // Input history of the interior column: planes pa-1, pa, pa+1 (registers).
-    float hin[3][kNumIn];
+    T hin[3][kNumIn];
     load_column<TX, TY>(f, qc0, uc0, qplane(z0 - 2), uplane(z0 - 2), hin[1]);
     load_column<TX, TY>(f, qc0, uc0, qplane(z0 - 1), uplane(z0 - 1), hin[2]);
     if (has_ring) {
-        float v[kNumIn];
+        T v[kNumIn];
         load_column<TX, TY>(f, qcr, ucr, qplane(z0 - 2), uplane(z0 - 2), v);
 #pragma unroll
         for (int m = 0; m < kNumIn; ++m) sRing[(z0 - 2 + 6) % 3][m][t] = v[m];
@@ -427,10 +441,10 @@ __device__ __forceinline__ void fused_body(const Fields<float>& f, const BCDesc<
     }
     // Ring and outer-ring inputs are loaded one plane ahead of their LDS
     // store: ring plane pa + 1 and outer plane pa arrive in iteration pa.
-    float rnext[kNumIn], onext[kNumIn];
+    T rnext[kNumIn], onext[kNumIn];
     if (has_ring) load_column<TX, TY>(f, qcr, ucr, qplane(z0), uplane(z0), rnext);
-    auto load_outer = [&](int z, float* v) {
-        const unsigned qo = (qplane(z) + (unsigned)oqc) * 4u, uo = (uplane(z) + (unsigned)ouc) * 4u;
+    auto load_outer = [&](int z, T* v) {
+        const unsigned qo = (qplane(z) + (unsigned)oqc) * E, uo = (uplane(z) + (unsigned)ouc) * E;
 #pragma unroll
         for (int m = 0; m < 5; ++m) v[m] = ldo(f.q[m], qo);
 #pragma unroll
@@ -441,10 +455,10 @@ __device__ __forceinline__ void fused_body(const Fields<float>& f, const BCDesc<
     // z-difference values of the interior column's stage A at plane pa - 2
     // (the lower z neighbour of the plane finished in step 3), and the
     // Ericksen force of the interior column for planes pa - 1 and pa.
-    float dz_back[kNumDZ];
+    T dz_back[kNumDZ];
 #pragma unroll
-    for (int m = 0; m < kNumDZ; ++m) dz_back[m] = 0.0f;
-    float fe_prev[3] = {0, 0, 0}, fe_cur[3] = {0, 0, 0};
+    for (int m = 0; m < kNumDZ; ++m) dz_back[m] = T(0);
+    T fe_prev[3] = {0, 0, 0}, fe_cur[3] = {0, 0, 0};
 
     for (int pa = z0 - 1; pa <= z1; ++pa) {
         // ---- step 1: input plane pa into LDS; history advance ----
@@ -460,7 +474,7 @@ __device__ __forceinline__ void fused_body(const Fields<float>& f, const BCDesc<
         // this thread writes, after the collision below.)
         const int pb = pa - 1;
         const unsigned cpb = (unsigned)pb * plane + (unsigned)gy * nx + gx;
-        float pop[kQ];
+        T pop[kQ];
         if (pb >= z0 && active) gather32<PERIODIC>(f, bc, gx, gy, pb, cpb, pull, pop);
         {
             const int cc = (ib + 1) * G::IX + ia + 1;
@@ -488,7 +502,7 @@ __device__ __forceinline__ void fused_body(const Fields<float>& f, const BCDesc<
         // ---- step 2: Q update (stage A) at plane pa ----
         const int par = (pa + 2) & 1;
         const int zq = map_coord(pa, nz, ax[2].qlo, ax[2].qhi), zu = map_coord(pa, nz, ax[2].ulo, ax[2].uhi);
-        float dz_fwd[kNumDZ];
+        T dz_fwd[kNumDZ];
         fe_prev[0] = fe_cur[0];
         fe_prev[1] = fe_cur[1];
         fe_prev[2] = fe_cur[2];
@@ -497,7 +511,7 @@ __device__ __forceinline__ void fused_body(const Fields<float>& f, const BCDesc<
         // the Q-update code serves both.
 #pragma unroll 1
         for (int slot = 0; slot < (has_ring ? 2 : 1); ++slot) {
-            float zm[kNumIn], c[kNumIn], zp[kNumIn];
+            T zm[kNumIn], c[kNumIn], zp[kNumIn];
             int qpos[3], upos[3], ci, cj, acol;
             if (slot == 0) {
 #pragma unroll
@@ -520,8 +534,8 @@ __device__ __forceinline__ void fused_body(const Fields<float>& f, const BCDesc<
             }
             qpos[2] = zq;
             upos[2] = zu;
-            StageA a;
-            float fe[3];
+            StageAT<T> a;
+            T fe[3];
             stage_a<TX, TY, PERIODIC>(bc, k, sIn, ci + 1, cj + 1, zm, c, zp, qpos, upos, a, fe);
 #pragma unroll
             for (int m = 0; m < kNumA; ++m) sA[par][m][acol] = stage_value(a, m);
@@ -537,7 +551,7 @@ __device__ __forceinline__ void fused_body(const Fields<float>& f, const BCDesc<
         __syncthreads();
 
         // ---- step 3: force + collide-stream at plane pb = pa - 1 ----
-        const float* A0 = &sA[(pb + 2) & 1][0][0];
+        const T* A0 = &sA[(pb + 2) & 1][0][0];
         auto A = [&](int m, int col) { return A0[m * G::NA + col]; };
         if (pb >= z0 && active) {
             // Neighbour columns with the reference's fetch rule: where a
@@ -548,34 +562,34 @@ __device__ __forceinline__ void fused_body(const Fields<float>& f, const BCDesc<
             const int lyp = (qy0 == ny - 1 && !ax[1].qhi) ? acell : acell + G::AX;
             const bool zm_self = (pb == 0 && !ax[2].qlo);
             const bool zp_self = (pb == nz - 1 && !ax[2].qhi);
-            auto zmv = [&](int m) -> float {
+            auto zmv = [&](int m) -> T {
                 const int sl = dz_slot(m);
                 return (zm_self || sl < 0) ? A(m, acell) : dz_back[sl];
             };
-            auto zpv = [&](int m) -> float {
+            auto zpv = [&](int m) -> T {
                 const int sl = dz_slot(m);
                 return (zp_self || sl < 0) ? A(m, acell) : dz_fwd[sl];
             };
             const int pos[3] = {qx0, qy0, pb};
-            float qc[5], dQ[5][3];
+            T qc[5], dQ[5][3];
 #pragma unroll
             for (int m = 0; m < 5; ++m) {
                 qc[m] = A(m, acell);
-                const Deriv<float> d = q_deriv<float, PERIODIC>(bc, m, pos, qc[m], A(m, lxm), A(m, lxp), A(m, lym), A(m, lyp),
+                const Deriv<T> d = q_deriv<T, PERIODIC>(bc, m, pos, qc[m], A(m, lxm), A(m, lxp), A(m, lym), A(m, lyp),
                                                zmv(m), zpv(m));
                 dQ[m][0] = d.d[0];
                 dQ[m][1] = d.d[1];
                 dQ[m][2] = d.d[2];
             }
-            float dS[8][3];
+            T dS[8][3];
 #pragma unroll
             for (int m = 0; m < 8; ++m) {
-                dS[m][0] = (A(5 + m, lxp) - A(5 + m, lxm)) / 2.0f;
-                dS[m][1] = (A(5 + m, lyp) - A(5 + m, lym)) / 2.0f;
-                dS[m][2] = (zpv(5 + m) - zmv(5 + m)) / 2.0f;
+                dS[m][0] = (A(5 + m, lxp) - A(5 + m, lxm)) / T(2.0);
+                dS[m][1] = (A(5 + m, lyp) - A(5 + m, lym)) / T(2.0);
+                dS[m][2] = (zpv(5 + m) - zmv(5 + m)) / T(2.0);
             }
-            const float uo[3] = {hin[0][5], hin[0][6], hin[0][7]};   // u^{n-1} at (gx, gy, pb)
-            float F[3];
+            const T uo[3] = {hin[0][5], hin[0][6], hin[0][7]};   // u^{n-1} at (gx, gy, pb)
+            T F[3];
             if (k.force_full) {
                 body_force(k, dQ, dS, uo, fe_prev, F);
             } else {
@@ -583,9 +597,12 @@ __device__ __forceinline__ void fused_body(const Fields<float>& f, const BCDesc<
                 F[1] = fe_prev[1];
                 F[2] = fe_prev[2];
             }
-            float drho, u[3];
-            collide_shifted(k, pop, F, drho, u);
-            const unsigned vs4 = (unsigned)f.vs * 4u, c4 = cpb * 4u;
+            // Density in its stored form: rho - rho0 for the shifted FP32
+            // populations, rho for FP64.
+            T drho, u[3];
+            if constexpr (sizeof(T) == 4) collide_shifted(k, pop, F, drho, u);
+            else collide_plain(k, pop, F, drho, u);
+            const unsigned vs4 = (unsigned)f.vs * E, c4 = cpb * E;
             if (wmask.writes(pb)) {
 #pragma unroll
                 for (int j = 0; j < kQ; ++j) sto(f.gn, j * vs4 + c4, pop[j]);
@@ -608,14 +625,14 @@ __device__ __forceinline__ void fused_body(const Fields<float>& f, const BCDesc<
     }
 }
 
-template <int TX, int TY, int ZC, bool PERIODIC>
+template <int TX, int TY, int ZC, bool PERIODIC, typename T = float>
 __global__ void __launch_bounds__(TX * TY)
-fused_step_kernel(Fields<float> f, BCDesc<float> bc, Phys<float> k, int pull, int aux, int ntx, int nty,
+fused_step_kernel(Fields<T> f, BCDesc<T> bc, Phys<T> k, int pull, int aux, int ntx, int nty,
                   int nzc, int mode) {
     using G = TileGeom<TX, TY>;
-    __shared__ float sIn[kNumIn][G::NI];          // input plane pa, extended region
-    __shared__ float sA[2][kNumA][G::NA];         // stage-A results, planes pa and pa - 1
-    __shared__ float sRing[3][kNumIn][G::NR];     // input history of the stage-A ring columns
+    __shared__ T sIn[kNumIn][G::NI];          // input plane pa, extended region
+    __shared__ T sA[2][kNumA][G::NA];         // stage-A results, planes pa and pa - 1
+    __shared__ T sRing[3][kNumIn][G::NR];     // input history of the stage-A ring columns
     // XCD-aware block order: hardware deals consecutive block ids round-robin
     // over the 8 XCDs; renumber so each XCD gets a contiguous run of tiles
     // (neighbouring tiles share halos in that XCD's L2).

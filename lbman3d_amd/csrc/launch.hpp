@@ -43,6 +43,15 @@ const char* fused_tile_name(int tile);
 void launch_fused_step_f32(const Fields<float>& f, const BCDesc<float>& bc, const Phys<float>& k,
                            int tile, int pull, int store_aux, hipStream_t s);
 
+// Fused single-kernel FP64 step on fully periodic domains: the first z-marching
+// pipeline with the unfused kernels' per-cell arithmetic, so its results are
+// those of launch_qstep + launch_force_lbm bit for bit (requires
+// 15 * vs * 8 < 2^32).
+int fused64_tile_count();
+const char* fused64_tile_name(int tile);
+void launch_fused_step_f64(const Fields<double>& f, const BCDesc<double>& bc, const Phys<double>& k, int tile,
+                           int pull, int store_aux, hipStream_t s);
+
 // Plain device-to-device copy kernel (HBM bandwidth calibration).
 void launch_copy(const void* src, void* dst, size_t bytes, hipStream_t s);
 

@@ -35,6 +35,7 @@ struct lbm_solver {
     int schedule = 0;      // FP32: 0 auto, 1 two-kernel, 2+k fused tile k
     int tuned = 0;         // schedule picked by lbm_autotune for "auto" (0: not tuned)
     bool fused_ok = false; // fused kernel usable (FP32, 32-bit offsets fit)
+    bool fused64_ok = false; // fused FP64 kernel usable (fully periodic, 32-bit offsets fit)
     bool moving = false;
 
     size_t esize() const { return fp64 ? sizeof(double) : sizeof(float); }

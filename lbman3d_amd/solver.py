@@ -84,6 +84,7 @@ SIGNATURES = {
     "lbm_closure_predict": (C.c_int, [C.c_void_p, _D, _D]),
     "lbm_closure_apply": (C.c_int, [C.c_void_p, C.c_void_p, C.c_double]),
     "lbm_closure_time_forward": (C.c_int, [C.c_void_p, C.c_int32, _D]),
+    "lbm_schedule_name": (C.c_int, [C.c_void_p, C.c_int32, C.c_char_p, C.c_size_t]),
     "lbm_closure_run": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_double, _D]),
     "lbm_debug_mfma_bf16": (C.c_int, [C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     "lbm_defect_faces": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint8), C.POINTER(C.c_uint8), C.POINTER(C.c_uint8)]),
@@ -262,6 +263,12 @@ class DeviceSolver:
 
     def set_schedule(self, s: int):
         check(lib().lbm_set_schedule(self.h, s))
+
+    def schedule_name(self, s: int) -> str:
+        """Name of schedule s for this handle's precision."""
+        buf = C.create_string_buffer(128)
+        check(lib().lbm_schedule_name(self.h, s, buf, 128))
+        return buf.value.decode()
 
     def get_schedule(self) -> int:
         s = C.c_int32()

@@ -1,0 +1,67 @@
+"""The fused FP64 step against the two-kernel FP64 step (needs a GPU).
+
+On fully periodic domains the FP64 step runs one z-marching kernel that keeps
+the Q-update results of the neighbouring columns on chip. It evaluates the
+same per-cell arithmetic as the two unfused kernels (no FMA contraction, the
+reference's operand order), so every tile shape must reproduce the two-kernel
+step bit for bit: rho, u, Q, the body force and the populations, on a grid
+whose sides are not multiples of any tile (partial tiles and z chunks), from
+an uploaded state (own-slot first step) through several step calls. The
+two-kernel step is itself pinned to the oracle bit for bit by the fixture
+tests; the 32^3 1000-step and 256^3 tests run the fused step as the default
+FP64 schedule against the oracle.
+"""
+import numpy as np
+import pytest
+
+from lbman3d_amd import abi
+
+pytestmark = pytest.mark.gpu
+
+DIMS = (37, 22, 45)
+
+
+def run(cfg, q, sched):
+    from lbman3d_amd.solver import DeviceSolver
+    with DeviceSolver(cfg) as s:
+        s.set_schedule(sched)
+        s.upload_state(q=q)
+        s.step(5)
+        s.step(7)
+        st = s.state()
+        st["F"] = s.force()
+        st["f"] = s.populations()
+        st["name"] = s.schedule_name(sched)
+    return st
+
+
+@pytest.mark.parametrize("upwind", [False, True])
+def test_every_fp64_tile_equals_the_two_kernel_step(upwind):
+    from lbman3d_amd.solver import DeviceSolver
+    cfg = abi.make_config(*DIMS, abi.FullyPeriodicConfig, abi.Params(ALPHA=0.046, LAMBDA=0.7 if upwind else 0.4),
+                          advection=abi.ADV_UPWIND if upwind else abi.ADV_CENTRED, precision=abi.FP64)
+    q = abi.seed_director_field(31, *DIMS)
+    ref = run(cfg, q, 1)
+    assert ref["name"] == "two-kernel"
+    with DeviceSolver(cfg) as s:
+        n = 0
+        while True:
+            try:
+                s.schedule_name(2 + n)
+            except Exception:
+                break
+            n += 1
+    assert n >= 1
+    for sched in [0] + list(range(2, 2 + n)):
+        st = run(cfg, q, sched)
+        print(f"case FP64 {st['name']} upwind={upwind}", flush=True)
+        assert st["name"] != "two-kernel" and "two-kernel" not in st["name"]
+        for k in ("rho", "u", "q", "F", "f"):
+            np.testing.assert_array_equal(st[k], ref[k], err_msg=f"{st['name']} {k}")
+
+
+def test_walled_fp64_domains_keep_the_two_kernel_step():
+    from lbman3d_amd.solver import DeviceSolver
+    cfg = abi.make_config(12, 10, 8, abi.ChannelConfig, abi.Params(), precision=abi.FP64)
+    with DeviceSolver(cfg) as s:
+        assert s.schedule_name(0) == "auto:two-kernel"
