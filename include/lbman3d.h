@@ -201,11 +201,11 @@ int lbm_time_steps(lbm_handle h, int32_t n, double* ms);
  * labels in the order of ms_out; at most max entries are written. */
 int lbm_profile_kernels(lbm_handle h, int32_t n, double* ms_out, int32_t max,
                         char* names, size_t names_len, int32_t* count);
-/* Kernel schedule. 0 = automatic, 1 = two-kernel split, 2.. = a fixed fused
- * tile shape. FP32: the automatic schedule is the autotuned fused
- * single-kernel step (tiles: lbm_tile_count / lbm_tile_name). FP64: fully
- * periodic domains run a fused single-kernel step whose results equal the
- * two-kernel split bit for bit; other domains the two-kernel split. */
+/* Kernel schedule. 0 = automatic (autotuned on first use), 1 = two-kernel
+ * split, 2.. = a fixed fused tile shape. FP32 tiles: lbm_tile_count /
+ * lbm_tile_name. FP64: fully periodic domains also have fused single-kernel
+ * tiles, whose results equal the two-kernel split bit for bit; other domains
+ * run the two-kernel split. */
 int lbm_set_schedule(lbm_handle h, int32_t schedule);
 int lbm_get_schedule(lbm_handle h, int32_t* schedule);
 int lbm_tile_count(void);
@@ -213,8 +213,9 @@ int lbm_tile_name(int32_t schedule, char* buf, size_t len);
 /* Name of a schedule of this handle's precision ("auto:...", "two-kernel",
  * "fused:..."). */
 int lbm_schedule_name(lbm_handle h, int32_t schedule, char* buf, size_t len);
-/* Time every FP32 schedule for n steps on a scratch copy of the state and
- * keep the fastest; results per schedule in ms_out[count] (ms per step). */
+/* Time every schedule of the handle's precision for n steps on a scratch copy
+ * of the state and keep the fastest for the automatic schedule; results per
+ * schedule (1, 2, ...) in ms_out[count] (ms per step). */
 int lbm_autotune(lbm_handle h, int32_t n, double* ms_out, int32_t max, int32_t* count);
 
 /* HBM calibration: device-to-device copy of `bytes` (16-byte accesses),

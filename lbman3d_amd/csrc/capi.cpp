@@ -32,8 +32,6 @@ using namespace lbm;
 // ---------------------------------------------------------------------------
 namespace lbm {
 thread_local std::string g_last_error;
-// FP64 fused tile used by the automatic schedule on fully periodic domains.
-constexpr int kDefaultTile64 = 1;   // fp64:32x4x128, the fastest at 256^3
 
 int fail(int code, const std::string& msg) {
     g_last_error = msg;
@@ -409,8 +407,9 @@ int run_lbm(lbm_solver* s, KernelRecorder* rec) {
 // Fused tile index for the next step, or -1 for the two-kernel split.
 int active_tile(const lbm_solver* s) {
     if (s->fp64) {
-        if (!s->fused64_ok || s->schedule == 1) return -1;
-        return s->schedule >= 2 ? s->schedule - 2 : kDefaultTile64;
+        if (!s->fused64_ok) return -1;
+        const int sched = s->schedule ? s->schedule : s->tuned;
+        return sched >= 2 ? sched - 2 : -1;
     }
     if (!s->fused_ok) return -1;
     int sched = s->schedule ? s->schedule : (s->tuned ? s->tuned : 2);
@@ -773,9 +772,13 @@ int lbm_lbm_step(lbm_handle s) {
 }  // extern "C"
 namespace lbm {
 void ensure_tuned(lbm_solver* s) {
-    if (s->fp64 || s->schedule != 0 || s->tuned != 0) return;
+    if (s->schedule != 0 || s->tuned != 0) return;
+    if (s->fp64 && !s->fused64_ok) {
+        s->tuned = 1;
+        return;
+    }
     const std::string saved_error = g_last_error;
-    if (lbm_autotune(s, 3, nullptr, 0, nullptr) != LBM_OK) s->tuned = s->fused_ok ? 2 : 1;
+    if (lbm_autotune(s, 3, nullptr, 0, nullptr) != LBM_OK) s->tuned = s->fp64 ? 1 : (s->fused_ok ? 2 : 1);
     g_last_error = saved_error;
 }
 }  // namespace lbm
@@ -961,7 +964,11 @@ int lbm_schedule_name(lbm_handle s, int32_t schedule, char* buf, size_t len) {
     if (!s || !buf || !len) return fail(LBM_ERR_INVALID, "null argument");
     if (!s->fp64) return lbm_tile_name(schedule, buf, len);
     std::string n;
-    if (schedule == 0) n = s->fused64_ok ? std::string("auto:") + fused64_tile_name(kDefaultTile64) : "auto:two-kernel";
+    if (schedule == 0) {
+        if (!s->fused64_ok || s->tuned == 1) n = "auto:two-kernel";
+        else if (s->tuned >= 2) n = std::string("auto:fused:") + fused64_tile_name(s->tuned - 2);
+        else n = "auto";
+    }
     else if (schedule == 1) n = "two-kernel";
     else if (schedule - 2 < fused64_tile_count()) n = std::string("fused:") + fused64_tile_name(schedule - 2);
     else return fail(LBM_ERR_INVALID, "schedule out of range");
@@ -970,13 +977,20 @@ int lbm_schedule_name(lbm_handle s, int32_t schedule, char* buf, size_t len) {
     return LBM_OK;
 }
 
-int lbm_autotune(lbm_handle s, int32_t n, double* ms_out, int32_t max, int32_t* count) {
-    if (!s || n < 1) return fail(LBM_ERR_INVALID, "bad argument");
-    if (count) *count = 0;
-    if (s->fp64) return fail(LBM_ERR_UNSUPPORTED, "the autotuner selects FP32 kernel schedules");
-    // Snapshot the evolving state so tuning leaves the simulation untouched.
-    Fields<float>& f = s->d32;
-    const size_t fb = s->V * sizeof(float), gb = kQ * s->vs * sizeof(float);
+}  // extern "C"
+namespace {
+// Schedule tuning on a snapshot of the evolving state (fields of type T), so
+// that it leaves the simulation as it was; the best schedule is kept in
+// s->tuned for the automatic schedule.
+template <typename T>
+int autotune_impl(lbm_solver* s, const std::vector<int>& cands, int32_t n, double* ms_out, int32_t max,
+                  int32_t* count) {
+    auto fields = [s]() -> Fields<T>& {
+        if constexpr (sizeof(T) == 4) return s->d32;
+        else return s->d64;
+    };
+    Fields<T>& f = fields();
+    const size_t fb = s->V * sizeof(T), gb = kQ * s->vs * sizeof(T);
     std::vector<std::pair<void*, size_t>> parts;
     for (int m = 0; m < 5; ++m) parts.push_back({f.q[m], fb});
     for (int m = 0; m < 3; ++m) parts.push_back({f.u[m], fb});
@@ -986,28 +1000,25 @@ int lbm_autotune(lbm_handle s, int32_t n, double* ms_out, int32_t max, int32_t* 
     for (auto& p : parts) total += p.second;
     void* snap = nullptr;
     HIPCHK(hipMalloc(&snap, total));
-    auto copy_all = [&](bool save) -> int {
+    {
         char* cur = (char*)snap;
-        HIPCHK(hipMemcpyAsync(save ? cur : (void*)f.g, save ? (void*)f.g : cur, gb, hipMemcpyDeviceToDevice, s->stream));
+        int r = LBM_OK;
+        if (hipMemcpyAsync(cur, f.g, gb, hipMemcpyDeviceToDevice, s->stream) != hipSuccess) r = LBM_ERR_HIP;
         cur += gb;
         for (auto& p : parts) {
-            HIPCHK(hipMemcpyAsync(save ? cur : p.first, save ? p.first : cur, p.second, hipMemcpyDeviceToDevice, s->stream));
+            if (hipMemcpyAsync(cur, p.first, p.second, hipMemcpyDeviceToDevice, s->stream) != hipSuccess) r = LBM_ERR_HIP;
             cur += p.second;
         }
-        return LBM_OK;
-    };
-    // The field pointers above are the current ones; snapshots restore into them.
+        if (r) {
+            (void)hipFree(snap);
+            return fail(LBM_ERR_HIP, "snapshot failed");
+        }
+    }
     const int pulled0 = s->pulled, sched0 = s->schedule;
-    int r = copy_all(true);
-    std::vector<int> cands = {1};
-    if (s->fused_ok)
-        for (int t = 0; t < fused_tile_count(); ++t) cands.push_back(2 + t);
     double best = 1e300;
-    int best_s = 1, k = 0;
+    int best_s = 1, k = 0, r = LBM_OK;
     for (int c : cands) {
         if (r) break;
-        // Restore pointers to the snapshot's owners: swaps during timing
-        // permute buffers, so rebuild the current set from the snapshot.
         s->schedule = c;
         s->pulled = pulled0;
         double ms = 0.0;
@@ -1017,8 +1028,9 @@ int lbm_autotune(lbm_handle s, int32_t n, double* ms_out, int32_t max, int32_t* 
         if (k < max && ms_out) ms_out[k] = ms;
         ++k;
         if (ms < best) { best = ms; best_s = c; }
-        // Put the original state back (into whatever buffers are now current).
-        Fields<float> now = s->d32;
+        // Put the original state back (into whatever buffers are now current:
+        // the steps permute the buffer pointers).
+        Fields<T> now = fields();
         char* cur = (char*)snap;
         if (hipMemcpyAsync(now.g, cur, gb, hipMemcpyDeviceToDevice, s->stream) != hipSuccess) { r = fail(LBM_ERR_HIP, "restore failed"); break; }
         cur += gb;
@@ -1036,6 +1048,22 @@ int lbm_autotune(lbm_handle s, int32_t n, double* ms_out, int32_t max, int32_t* 
     s->tuned = best_s;
     if (count) *count = std::min(k, (int)max);
     return LBM_OK;
+}
+}  // namespace
+extern "C" {
+
+int lbm_autotune(lbm_handle s, int32_t n, double* ms_out, int32_t max, int32_t* count) {
+    if (!s || n < 1) return fail(LBM_ERR_INVALID, "bad argument");
+    if (count) *count = 0;
+    std::vector<int> cands = {1};
+    if (s->fp64) {
+        if (s->fused64_ok)
+            for (int t = 0; t < fused64_tile_count(); ++t) cands.push_back(2 + t);
+        return autotune_impl<double>(s, cands, n, ms_out, max, count);
+    }
+    if (s->fused_ok)
+        for (int t = 0; t < fused_tile_count(); ++t) cands.push_back(2 + t);
+    return autotune_impl<float>(s, cands, n, ms_out, max, count);
 }
 
 int lbm_bandwidth_probe(size_t bytes, int32_t reps, double* gbs, double* ms_per_copy) {

@@ -279,7 +279,7 @@ class DeviceSolver:
         ms = (C.c_double * 64)()
         cnt = C.c_int32()
         check(lib().lbm_autotune(self.h, n, ms, 64, C.byref(cnt)))
-        return {schedule_name(i + 1): ms[i] for i in range(cnt.value)}
+        return {self.schedule_name(i + 1): ms[i] for i in range(cnt.value)}
 
 
 def tile_count() -> int:
