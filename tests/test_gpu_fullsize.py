@@ -8,9 +8,9 @@
   the flow turns chaotic.
 - FullyPeriodic 256^3, zeta 0.046, lambda 0.7, upwind: the bandwidth
   workload. FP64 must equal the oracle bit for bit after 5 steps. The FP32
-  step on the schedules the bandwidth run uses (p2 and p2z, 64 x 4 x 128
-  tiles, two z chunks per column, so the chunk boundary at z = 128 is
-  crossed) must stay within 1e-6 of the oracle after those 5 steps, and
+  step on the schedules the bandwidth run uses (p2, p2z and their forms with
+  unconditional memory instructions, 64 x 4 x 128 tiles, two z chunks per
+  column, so the chunk boundary at z = 128 is crossed) must stay within 1e-6 of the oracle after those 5 steps, and
   within 1e-5 of the FP64 step after 250 steps (the FP64 step at this size
   being the oracle's bits, by the first check).
 
@@ -122,7 +122,7 @@ def test_c2_fp64_bitwise_5_steps(c2_oracle_5, c2_q):
         np.testing.assert_array_equal(st[k], c2_oracle_5[k], err_msg=k)
 
 
-@pytest.mark.parametrize("sched", [None, "p2:64x4x128", "p2z:64x4x128"])
+@pytest.mark.parametrize("sched", [None, "p2:64x4x128", "p2z:64x4x128", "p2u:64x4x128", "p2uz:64x4x128"])
 def test_c2_fp32_bench_schedules_against_oracle(c2_oracle_5, c2_q, sched):
     print(f"case C2 FP32 5 steps schedule={sched}", flush=True)
     st = device_run(c2_config(abi.FP32_SHIFTED), c2_q, 5, None if sched is None else schedule(sched))
@@ -131,7 +131,7 @@ def test_c2_fp32_bench_schedules_against_oracle(c2_oracle_5, c2_q, sched):
     assert e["q"] < 1e-6 and e["u"] < 1e-5 and e["rho"] < 1e-5, e
 
 
-@pytest.mark.parametrize("sched", ["p2:64x4x128", "p2z:64x4x128"])
+@pytest.mark.parametrize("sched", ["p2:64x4x128", "p2z:64x4x128", "p2uz:64x4x128"])
 def test_c2_fp32_against_fp64_after_250_steps(c2_q, sched):
     print(f"case C2 FP32 vs FP64 250 steps schedule={sched}", flush=True)
     ref = device_run(c2_config(abi.FP64), c2_q, 250)

@@ -26,7 +26,10 @@ CASES = [("FullyPeriodic", (37, 29, 23), abi.ADV_UPWIND), ("Channel", (40, 20, 1
          # overlapping shells: every cell within two cells of a wall
          ("Channel", (3, 5, 10), abi.ADV_CENTRED),
          # thin in z: one and two planes (the z march wraps onto the same planes)
-         ("FullyPeriodic", (40, 24, 2), abi.ADV_UPWIND), ("Channel", (24, 20, 1), abi.ADV_CENTRED)]
+         ("FullyPeriodic", (40, 24, 2), abi.ADV_UPWIND), ("Channel", (24, 20, 1), abi.ADV_CENTRED),
+         # whole tiles in x and y: the schedules whose memory instructions are
+         # unconditional run only on such grids
+         ("FullyPeriodic", (128, 64, 40), abi.ADV_UPWIND)]
 
 
 def rel(a, b):
