@@ -34,6 +34,7 @@ using namespace lbm;
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef uint16_t u16x8 __attribute__((ext_vector_type(8)));
 
 namespace {
 
@@ -64,17 +65,20 @@ __device__ __forceinline__ int wrap(int t, int n) { return t < 0 ? t + n : (t >=
 //
 // SRC: 0 = BF16 input [cell][CIN]; 1 = the closure input read straight from a
 // simulation's FP32 fields (Q 5, u 3) times in_scale (Cin 8).
-// EPI: 0 = store BF16 with bias + ReLU, 1 = store BF16 linear (no bias),
+// EPI: 0 = store BF16 with bias + ReLU,
 // 2 = store FP32 channels-first [co][cell] with bias (final layer),
 // 3 = add alpha * (value + bias) / out_scale to the simulation's Q (final
-// layer applied in place).
+// layer applied in place), 4 = store BF16 linear, zero where the stored
+// activation `act` of the same cell and channel is not positive (the input
+// gradient through a ReLU).
 constexpr int kTX = 16, kTY = 4;
 
 struct ConvIO {
     const __bf16* in;          // SRC 0
     const float* fin[8];       // SRC 1: Qxx, Qxy, Qxz, Qyy, Qyz, ux, uy, uz
     const float* in_scale;     // SRC 1
-    void* out;                 // EPI 0-2
+    void* out;                 // EPI 0-2, 4
+    const __bf16* act;         // EPI 4: [cell][cout]
     float* q[5];               // EPI 3
     const float* out_scale;    // EPI 3
     float alpha;               // EPI 3
@@ -151,7 +155,7 @@ conv_fwd_kernel(ConvIO io, const bf16x8* __restrict__ wpk, const float* __restri
     for (int n = 0; n < NT; ++n) {
         const int co = n * 16 + m;
         if (co >= cout) continue;
-        const float bco = (EPI == 1) ? 0.f : bias[co];
+        const float bco = EPI == 4 ? 0.f : bias[co];
 #pragma unroll
         for (int t = 0; t < kTY; ++t) {
             const long row = ((long)(bz * WZ + w) * ny + by * kTY + t) * nx + bx * kTX;
@@ -160,6 +164,7 @@ conv_fwd_kernel(ConvIO io, const bf16x8* __restrict__ wpk, const float* __restri
                 const long cell = row + 4 * kb + r;
                 float v = acc[t][n][r] + bco;
                 if (EPI == 0) v = v > 0.f ? v : 0.f;
+                if (EPI == 4 && !(from_bf16(io.act[cell * cout + co]) > 0.f)) v = 0.f;
                 if (EPI == 3) io.q[co][cell] += io.alpha * v / io.out_scale[co];
                 else if (EPI == 2) reinterpret_cast<float*>(io.out)[(long)co * V + cell] = v;
                 else reinterpret_cast<__bf16*>(io.out)[cell * cout + co] = to_bf16(v);
@@ -168,114 +173,192 @@ conv_fwd_kernel(ConvIO io, const bf16x8* __restrict__ wpk, const float* __restri
     }
 }
 
-// ---- weight gradient --------------------------------------------------------
-// dW[k][co] = sum_cell A[cell][k] dOut[cell][co], k = off * CIN + ci. One
-// wavefront owns KT 16-row k tiles and all NT 16-column co tiles over a chunk
-// of cells, so each dOut fragment serves KT tiles and each gathered input
-// fragment NT tiles. A lane's 8 cells of a 32-cell step are consecutive and
-// lie in one x row (nx is a multiple of 8), so the neighbour row is found
-// once per step. Partial sums land in part[chunk][k][co] and are reduced by
-// wgrad_reduce_kernel.
-template <int CIN, int KT, int NT>
-__global__ void __launch_bounds__(256)
-conv_wgrad_kernel(const __bf16* __restrict__ in, const __bf16* __restrict__ dout, int cout_pad, int cout,
-                  float* __restrict__ part, int ktiles, int chunk, int nx, int ny, int nz) {
-    const int lane = threadIdx.x & 63;
-    const long wave = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const long V = (long)nx * ny * nz;
-    const int nchunks = (int)((V + chunk - 1) / chunk);
-    const int kgroups = (ktiles + KT - 1) / KT;
-    if (wave >= (long)kgroups * nchunks) return;
-    const int ch_id = (int)(wave / kgroups);
-    const int kg = (int)(wave % kgroups);
+// ---- weight and bias gradient ---------------------------------------------
+// dW[off][ci][co] = sum_cell in[cell + d(off)][ci] dOut[cell][co] and
+// db[co] = sum_cell dOut[cell][co]: a GEMM whose K index runs over cells.
+// A block of nine wavefronts takes 16 x 4 x 2-cell tiles in turn. For each
+// tile it stages the periodic 20 x 6 x 4 input halo and the dOut tile in LDS
+// channel-major (x fastest, one row per (z, y)), transposing the [cell][C]
+// vectors of global memory on the way in. Wavefront w owns the three stencil
+// offsets (dx = -1, 0, 1) of one (dy, dz) = (w % 3 - 1, w / 3 - 1): per
+// 32-cell K step (two 16-cell rows) a lane reads the 8 cells of its row
+// segment as one 16-byte LDS vector plus the dword on each side, and forms
+// the fragments of the three x shifts with 16-bit funnel shifts
+// (v_alignbit_b32), so every LDS read stays aligned. The dOut fragments are
+// shared by the 3 x (CIN / 16) A fragments of the step. Wavefront 4
+// (dy = dz = 0) also multiplies an all-ones A fragment into its dOut
+// fragments: the bias gradient. A block accumulates over its tiles in
+// registers and writes one partial [27 CIN + 1][cout] (weights, then the bias
+// row); wgrad_reduce_kernel adds the partials of all blocks.
+constexpr int kWgTX = 16, kWgTY = 4, kWgTZ = 2;
+constexpr int kWgRL = 24;                                     // bf16 per LDS row (x0 - 2 .. x0 + 17, padded)
+constexpr int kWgHY = kWgTY + 2, kWgHZ = kWgTZ + 2;
+constexpr int kWgAStr = kWgHZ * kWgHY * kWgRL + 8;            // per input channel: 73 x 16 B (odd: no bank clash)
+constexpr int kWgBStr = kWgTZ * kWgTY * kWgTX + 8;            // per output channel: 17 x 16 B
+constexpr int kWgWaves = 9;
+
+template <int CIN, int CS>
+__global__ void __launch_bounds__(64 * kWgWaves)
+conv_wgrad_kernel(const __bf16* __restrict__ in, const __bf16* __restrict__ dout, int cout,
+                  float* __restrict__ part, int ntiles, int per_block, int nx, int ny, int nz) {
+    constexpr int NTI = (CIN + 15) / 16, NTO = (CS + 15) / 16;
+    constexpr int CVI = CIN / 8, CVO = CS / 8;
+    __shared__ __attribute__((aligned(16))) __bf16 sA[CIN * kWgAStr + 8];
+    __shared__ __attribute__((aligned(16))) __bf16 sB[CS * kWgBStr];
+    uint32_t* sA32 = reinterpret_cast<uint32_t*>(sA);
+    uint32_t* sB32 = reinterpret_cast<uint32_t*>(sB);
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int m = lane & 15, kb = lane >> 4;
-    int ci[KT], dx[KT], dy[KT], dz[KT];
-    bool valid[KT];
+    const int dz = w / 3 - 1, dy = w % 3 - 1;
+    const int ntx = nx / kWgTX;
+    const int t0 = blockIdx.x * per_block, t1 = min(t0 + per_block, ntiles);
+
+    f32x4 acc[3][NTI][NTO], accb[NTO];
 #pragma unroll
-    for (int t = 0; t < KT; ++t) {
-        const int kt = kg * KT + t;
-        const int k = kt * 16 + m;
-        const int off = k / CIN;
-        ci[t] = k % CIN;
-        valid[t] = kt < ktiles && off < kOffsets;
-        dz[t] = off / 9 - 1;
-        dy[t] = (off / 3) % 3 - 1;
-        dx[t] = off % 3 - 1;
-    }
-    f32x4 acc[KT][NT];
+    for (int d = 0; d < 3; ++d)
 #pragma unroll
-    for (int t = 0; t < KT; ++t)
+        for (int t = 0; t < NTI; ++t)
 #pragma unroll
-        for (int n = 0; n < NT; ++n) acc[t][n] = (f32x4){0.f, 0.f, 0.f, 0.f};
-    const long cbeg = (long)ch_id * chunk, cend = cbeg + chunk < V ? cbeg + chunk : V;
-    for (long cb = cbeg; cb < cend; cb += 32) {
-        const long g0 = cb + 8 * kb;
-        const bool gok = g0 < cend;
-        const int x0 = (int)(g0 % nx), y = (int)((g0 / nx) % ny), z = (int)(g0 / ((long)nx * ny));
-        bf16x8 b[NT];
+            for (int n = 0; n < NTO; ++n) acc[d][t][n] = (f32x4){0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int n = 0; n < NT; ++n) {
-            const int co = n * 16 + m;
+    for (int n = 0; n < NTO; ++n) accb[n] = (f32x4){0.f, 0.f, 0.f, 0.f};
+    bf16x8 ones;
 #pragma unroll
-            for (int j = 0; j < 8; ++j) b[n][j] = (gok && co < cout) ? dout[(g0 + j) * cout_pad + co] : to_bf16(0.f);
+    for (int j = 0; j < 8; ++j) ones[j] = to_bf16(1.f);
+
+    for (int tile = t0; tile < t1; ++tile) {
+        // tiles z-fastest, so a block's consecutive tiles share halo planes in L2
+        const int tz = tile % (nz / kWgTZ), txy = tile / (nz / kWgTZ);
+        const int x0 = (txy % ntx) * kWgTX, y0 = (txy / ntx) * kWgTY, z0 = tz * kWgTZ;
+        __syncthreads();   // the previous tile's LDS reads are done
+        // input halo: 10 cell pairs per row x 24 rows x CVI vectors
+        for (int it = tid; it < 10 * kWgHY * kWgHZ * CVI; it += 64 * kWgWaves) {
+            const int cv = it % CVI, pr = it / CVI;
+            const int p = pr % 10, row = pr / 10;
+            const int hy = row % kWgHY, hz = row / kWgHY;
+            const long rbase = ((long)wrap(z0 - 1 + hz, nz) * ny + wrap(y0 - 1 + hy, ny)) * nx;
+            const int xa = wrap(x0 - 2 + 2 * p, nx), xb = wrap(x0 - 1 + 2 * p, nx);
+            const u16x8 va = *reinterpret_cast<const u16x8*>(in + (rbase + xa) * CIN + cv * 8);
+            const u16x8 vb = *reinterpret_cast<const u16x8*>(in + (rbase + xb) * CIN + cv * 8);
+            const int e = 8 + row * kWgRL + 2 * p - 2;   // element index of x0 - 2 + 2p in channel 0
+#pragma unroll
+            for (int j = 0; j < 8; ++j) sA32[((cv * 8 + j) * kWgAStr + e) >> 1] = (uint32_t)va[j] | ((uint32_t)vb[j] << 16);
         }
+        // dOut tile: 8 cell pairs per row x 8 rows x CVO vectors
+        for (int it = tid; it < 8 * kWgTY * kWgTZ * CVO; it += 64 * kWgWaves) {
+            const int cv = it % CVO, pr = it / CVO;
+            const int p = pr % 8, row = pr / 8;
+            const int yy = row % kWgTY, zz = row / kWgTY;
+            const long c = ((long)(z0 + zz) * ny + y0 + yy) * nx + x0 + 2 * p;
+            const u16x8 va = *reinterpret_cast<const u16x8*>(dout + c * CS + cv * 8);
+            const u16x8 vb = *reinterpret_cast<const u16x8*>(dout + (c + 1) * CS + cv * 8);
+            const int e = row * kWgTX + 2 * p;
 #pragma unroll
-        for (int t = 0; t < KT; ++t) {
-            bf16x8 a;
-            if (gok && valid[t]) {
-                const long row = ((long)wrap(z + dz[t], nz) * ny + wrap(y + dy[t], ny)) * nx;
+            for (int j = 0; j < 8; ++j) sB32[((cv * 8 + j) * kWgBStr + e) >> 1] = (uint32_t)va[j] | ((uint32_t)vb[j] << 16);
+        }
+        __syncthreads();
 #pragma unroll
-                for (int j = 0; j < 8; ++j) a[j] = in[(row + wrap(x0 + j + dx[t], nx)) * CIN + ci[t]];
-            } else {
+        for (int s = 0; s < kWgTY * kWgTZ / 2; ++s) {
+            const int rr = 2 * s + (kb >> 1), xo = 8 * (kb & 1);
+            const int yy = rr % kWgTY, zz = rr / kWgTY;
+            bf16x8 b[NTO];
 #pragma unroll
-                for (int j = 0; j < 8; ++j) a[j] = to_bf16(0.f);
+            for (int n = 0; n < NTO; ++n) {
+                const int co = 16 * n + m;
+                if (co < CS) {
+                    b[n] = *reinterpret_cast<const bf16x8*>(sB + co * kWgBStr + rr * kWgTX + xo);
+                } else {
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) b[n][j] = to_bf16(0.f);
+                }
             }
 #pragma unroll
-            for (int n = 0; n < NT; ++n) acc[t][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[n], acc[t][n], 0, 0, 0);
+            for (int t = 0; t < NTI; ++t) {
+                const int ci = 16 * t + m;
+                typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+                u32x4 C = (u32x4){0u, 0u, 0u, 0u};
+                uint32_t L = 0u, R = 0u;
+                if (ci < CIN) {
+                    const int e = ci * kWgAStr + 8 + ((zz + dz + 1) * kWgHY + (yy + dy + 1)) * kWgRL + xo;
+                    C = *reinterpret_cast<const u32x4*>(sA + e);
+                    L = sA32[(e - 2) >> 1];
+                    R = sA32[(e + 8) >> 1];
+                }
+                const u32x4 Am = (u32x4){__builtin_amdgcn_alignbit(C[0], L, 16), __builtin_amdgcn_alignbit(C[1], C[0], 16),
+                                         __builtin_amdgcn_alignbit(C[2], C[1], 16), __builtin_amdgcn_alignbit(C[3], C[2], 16)};
+                const u32x4 Ap = (u32x4){__builtin_amdgcn_alignbit(C[1], C[0], 16), __builtin_amdgcn_alignbit(C[2], C[1], 16),
+                                         __builtin_amdgcn_alignbit(C[3], C[2], 16), __builtin_amdgcn_alignbit(R, C[3], 16)};
+                const bf16x8 a0 = __builtin_bit_cast(bf16x8, Am), a1 = __builtin_bit_cast(bf16x8, C),
+                             a2 = __builtin_bit_cast(bf16x8, Ap);
+#pragma unroll
+                for (int n = 0; n < NTO; ++n) {
+                    acc[0][t][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b[n], acc[0][t][n], 0, 0, 0);
+                    acc[1][t][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b[n], acc[1][t][n], 0, 0, 0);
+                    acc[2][t][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2, b[n], acc[2][t][n], 0, 0, 0);
+                }
+            }
+            if (w == 4) {
+#pragma unroll
+                for (int n = 0; n < NTO; ++n)
+                    accb[n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ones, b[n], accb[n], 0, 0, 0);
+            }
         }
     }
-    // D: rows k = kt*16 + 4*kb + r, column co = n*16 + m.
-    const int krows = ktiles * 16, ncols = NT * 16;
+    // D fragment: lane (m, kb) holds rows 4 kb + r (input channel within the
+    // ci tile) of column m (output channel within the co tile).
+    const int krows = 27 * CIN + 1;
+    float* pb = part + (long)blockIdx.x * krows * cout;
 #pragma unroll
-    for (int t = 0; t < KT; ++t) {
-        const int kt = kg * KT + t;
-        if (kt >= ktiles) continue;
+    for (int d = 0; d < 3; ++d) {
+        const int off = (dz + 1) * 9 + (dy + 1) * 3 + d;
 #pragma unroll
-        for (int n = 0; n < NT; ++n)
+        for (int t = 0; t < NTI; ++t)
 #pragma unroll
-            for (int r = 0; r < 4; ++r)
-                part[((long)ch_id * krows + kt * 16 + 4 * kb + r) * ncols + n * 16 + m] = acc[t][n][r];
+            for (int n = 0; n < NTO; ++n)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int ci = 16 * t + 4 * kb + r, co = 16 * n + m;
+                    if (ci < CIN && co < cout) pb[(off * CIN + ci) * cout + co] = acc[d][t][n][r];
+                }
+    }
+    if (w == 4 && kb == 0) {
+#pragma unroll
+        for (int n = 0; n < NTO; ++n) {
+            const int co = 16 * n + m;
+            if (co < cout) pb[(krows - 1) * cout + co] = accb[n][0];
+        }
     }
 }
 
-__global__ void wgrad_reduce_kernel(const float* __restrict__ part, int nchunks, int krows, int ncols, int kvalid,
-                                    int cvalid, float* __restrict__ grad) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= kvalid * cvalid) return;
-    const int k = i / cvalid, co = i % cvalid;
-    float s = 0.f;
-    for (int c = 0; c < nchunks; ++c) s += part[((long)c * krows + k) * ncols + co];
-    grad[i] = s;   // [k][co] = [off][ci][co]
-}
-
-// Bias gradient, first pass: per-block sums of dOut[cell][co] over a range
-// of cells (cpad, the stored channel count, divides 256); the per-block
-// partials are then added by wgrad_reduce_kernel.
+// Sum of the per-block partials [block][27 cin + 1][cout]: the weight rows go
+// to gw ([off][ci][co]), the last row to gb. A block of 256 threads owns 64
+// consecutive outputs; its four wavefronts add every fourth partial (eight
+// loads in flight per lane), and LDS adds the four sums.
 __global__ void __launch_bounds__(256)
-bias_partial_kernel(const __bf16* __restrict__ dout, int cpad, long V, float* __restrict__ part) {
-    __shared__ float red[256];
-    const int t = threadIdx.x;
-    const int co = t % cpad, sub = t / cpad, step = 256 / cpad;
-    const long per = (V + gridDim.x - 1) / gridDim.x;
-    const long beg = (long)blockIdx.x * per, end = beg + per < V ? beg + per : V;
+wgrad_reduce_kernel(const float* __restrict__ part, int nblocks, int krows, int cout, float* __restrict__ gw,
+                    float* __restrict__ gb) {
+    __shared__ float red[4][64];
+    const int lane = threadIdx.x & 63, g = threadIdx.x >> 6;
+    const long n = (long)krows * cout;
+    const long i = (long)blockIdx.x * 64 + lane;
     float s = 0.f;
-    for (long c = beg + sub; c < end; c += step) s += from_bf16(dout[c * cpad + co]);
-    red[t] = s;
+    if (i < n) {
+        int b = g;
+        for (; b + 28 < nblocks; b += 32) {
+            float v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) v[u] = part[(long)(b + 4 * u) * n + i];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) s += v[u];
+        }
+        for (; b < nblocks; b += 4) s += part[(long)b * n + i];
+    }
+    red[g][lane] = s;
     __syncthreads();
-    if (t < cpad) {
-        float a = 0.f;
-        for (int j = t; j < 256; j += cpad) a += red[j];
-        part[(long)blockIdx.x * cpad + t] = a;
+    if (g == 0 && i < n) {
+        const float t = (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
+        if (i < (long)(krows - 1) * cout) gw[i] = t;
+        else gb[i - (long)(krows - 1) * cout] = t;
     }
 }
 
@@ -309,20 +392,30 @@ loss_grad_kernel(const float* __restrict__ pred, const float* __restrict__ targe
     if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
 }
 
-// ReLU backward in place: gradient is zero where the stored activation is zero.
-__global__ void relu_back_kernel(__bf16* __restrict__ g, const __bf16* __restrict__ act, long n) {
-    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n && !(from_bf16(act[i]) > 0.f)) g[i] = to_bf16(0.f);
-}
-
 // Pack FP32 master weights W[off][ci][co] into B fragments for a
 // convolution kernel whose input has kcin channels per cell (K index
 // off * kcin + c). transpose: pack the input-gradient weights
-// W'[off][co][ci] = W[26 - off][ci][co].
-__global__ void pack_kernel(const float* __restrict__ w, int cin, int cout, int transpose, int kcin, int kch,
-                            int ntiles, bf16x8* __restrict__ out) {
-    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;   // (kc, t, lane)
-    if (idx >= (long)kch * ntiles * 64) return;
+// W'[off][co][ci] = W[26 - off][ci][co]. One launch packs every job of the
+// list (the forward fragments of the three layers and the input-gradient
+// fragments of layers 2 and 3); job j covers the thread range
+// [start[j], start[j + 1]).
+struct PackJobs {
+    const float* w[5];
+    bf16x8* out[5];
+    int cin[5], cout[5], transpose[5], kcin[5], ntiles[5];
+    long start[6];
+    int njobs;
+};
+
+__global__ void pack_kernel(PackJobs jobs) {
+    const long gidx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gidx >= jobs.start[jobs.njobs]) return;
+    int jb = 0;
+    while (gidx >= jobs.start[jb + 1]) ++jb;
+    const long idx = gidx - jobs.start[jb];   // (kc, t, lane)
+    const int ntiles = jobs.ntiles[jb], kcin = jobs.kcin[jb], cin = jobs.cin[jb], cout = jobs.cout[jb];
+    const int transpose = jobs.transpose[jb];
+    const float* w = jobs.w[jb];
     const int lane = (int)(idx % 64), t = (int)((idx / 64) % ntiles), kc = (int)(idx / (64 * ntiles));
     const int ecin = transpose ? cout : cin, ecout = transpose ? cin : cout;
     const int n = t * 16 + (lane & 15);
@@ -337,7 +430,7 @@ __global__ void pack_kernel(const float* __restrict__ w, int cin, int cout, int 
         }
         v[j] = to_bf16(val);
     }
-    out[idx] = v;
+    jobs.out[jb][idx] = v;
 }
 
 __global__ void adam_kernel(float* __restrict__ w, const float* __restrict__ g, float* __restrict__ m,
@@ -403,7 +496,7 @@ struct lbm_closure {
     std::vector<__bf16*> sx;   // stored samples: inputs
     std::vector<float*> sy;    // stored samples: targets [5][cell] normalised
     long step = 0;
-    int chunk = 4096;
+    int wg_blocks = 256;   // weight-gradient blocks (partials) per layer
 };
 
 namespace {
@@ -425,19 +518,27 @@ int cl_alloc(lbm_closure* c, T** p, size_t elems) {
 }
 
 int repack(lbm_closure* c) {
-    for (int l = 0; l < 3; ++l) {
-        const long n = (long)c->kch[l] * c->nt[l] * 64;
-        hipLaunchKernelGGL(pack_kernel, dim3(blocks_for(n)), dim3(256), 0, c->stream, c->w + c->off_w[l], c->cin[l],
-                           c->cout[l], 0, c->cin[l], c->kch[l], c->nt[l], c->pk[l]);
-    }
-    for (int l = 1; l < 3; ++l) {
-        const long n = (long)c->kcht[l - 1] * c->ntt[l - 1] * 64;
-        // input of the transposed convolution: the layer's output gradient,
-        // stored with 8 channels (padded) for the last layer, else cout
-        const int kcin = c->cout[l] <= 8 ? 8 : c->cout[l];
-        hipLaunchKernelGGL(pack_kernel, dim3(blocks_for(n)), dim3(256), 0, c->stream, c->w + c->off_w[l], c->cin[l],
-                           c->cout[l], 1, kcin, c->kcht[l - 1], c->ntt[l - 1], c->pkt[l - 1]);
-    }
+    PackJobs jobs{};
+    long n = 0;
+    auto add = [&](int l, int transpose, int kcin, int kch, int ntiles, bf16x8* out) {
+        const int j = jobs.njobs++;
+        jobs.w[j] = c->w + c->off_w[l];
+        jobs.out[j] = out;
+        jobs.cin[j] = c->cin[l];
+        jobs.cout[j] = c->cout[l];
+        jobs.transpose[j] = transpose;
+        jobs.kcin[j] = kcin;
+        jobs.ntiles[j] = ntiles;
+        jobs.start[j] = n;
+        n += (long)kch * ntiles * 64;
+        jobs.start[j + 1] = n;
+    };
+    for (int l = 0; l < 3; ++l) add(l, 0, c->cin[l], c->kch[l], c->nt[l], c->pk[l]);
+    // input of the transposed convolution: the layer's output gradient,
+    // stored with 8 channels (padded) for the last layer, else cout
+    for (int l = 1; l < 3; ++l)
+        add(l, 1, c->cout[l] <= 8 ? 8 : c->cout[l], c->kcht[l - 1], c->ntt[l - 1], c->pkt[l - 1]);
+    hipLaunchKernelGGL(pack_kernel, dim3(blocks_for(n)), dim3(256), 0, c->stream, jobs);
     CHK(hipGetLastError());
     return LBM_OK;
 }
@@ -492,37 +593,30 @@ int forward_apply(lbm_closure* c, const Fields<float>& f, double alpha, hipStrea
     return LBM_OK;
 }
 
-int wgrad(lbm_closure* c, const __bf16* in, int cin, const __bf16* dout, int cout_pad, int cout, float* gw) {
-    constexpr int KT = 2;
-    const int krows = ((cin == 8 ? 28 : 27) * cin + 15) / 16 * 16;
-    const int ktiles = krows / 16, ntiles = (cout + 15) / 16;
-    const int nchunks = (int)((c->V + c->chunk - 1) / c->chunk);
-    const long waves = (long)((ktiles + KT - 1) / KT) * nchunks;
-    if ((long)nchunks * krows * ntiles * 16 > c->part_elems) return fail(LBM_ERR_STATE, "wgrad scratch too small");
-    const dim3 grid(blocks_for(waves * 64));
-    if (cin == 8 && ntiles == 2)
-        hipLaunchKernelGGL((conv_wgrad_kernel<8, KT, 2>), grid, dim3(256), 0, c->stream, in, dout, cout_pad, cout,
-                           c->part, ktiles, c->chunk, c->nx, c->ny, c->nz);
-    else if (cin == 32 && ntiles == 2)
-        hipLaunchKernelGGL((conv_wgrad_kernel<32, KT, 2>), grid, dim3(256), 0, c->stream, in, dout, cout_pad, cout,
-                           c->part, ktiles, c->chunk, c->nx, c->ny, c->nz);
-    else if (cin == 32 && ntiles == 1)
-        hipLaunchKernelGGL((conv_wgrad_kernel<32, KT, 1>), grid, dim3(256), 0, c->stream, in, dout, cout_pad, cout,
-                           c->part, ktiles, c->chunk, c->nx, c->ny, c->nz);
+// Weight and bias gradient of one layer (input `in` with cin channels per
+// cell, output gradient `dout` with cs stored channels of which cout are
+// real) into gw [off][ci][co] and gb [co].
+int wgrad(lbm_closure* c, const __bf16* in, int cin, const __bf16* dout, int cs, int cout, float* gw, float* gb) {
+    const int ntiles = (int)(c->V / (kWgTX * kWgTY * kWgTZ));
+    const int nblk = std::min(ntiles, c->wg_blocks);
+    const int per = (ntiles + nblk - 1) / nblk;
+    const int blocks = (ntiles + per - 1) / per;
+    const int krows = 27 * cin + 1;
+    if ((long)blocks * krows * cout > c->part_elems) return fail(LBM_ERR_STATE, "wgrad scratch too small");
+    const dim3 grid(blocks), block(64 * kWgWaves);
+    if (cin == 8 && cs == 32)
+        hipLaunchKernelGGL((conv_wgrad_kernel<8, 32>), grid, block, 0, c->stream, in, dout, cout, c->part, ntiles, per,
+                           c->nx, c->ny, c->nz);
+    else if (cin == 32 && cs == 32)
+        hipLaunchKernelGGL((conv_wgrad_kernel<32, 32>), grid, block, 0, c->stream, in, dout, cout, c->part, ntiles,
+                           per, c->nx, c->ny, c->nz);
+    else if (cin == 32 && cs == 8)
+        hipLaunchKernelGGL((conv_wgrad_kernel<32, 8>), grid, block, 0, c->stream, in, dout, cout, c->part, ntiles, per,
+                           c->nx, c->ny, c->nz);
     else
         return fail(LBM_ERR_UNSUPPORTED, "closure layer shape");
-    const int kvalid = 27 * cin;
-    hipLaunchKernelGGL(wgrad_reduce_kernel, dim3(blocks_for((long)kvalid * cout)), dim3(256), 0, c->stream, c->part,
-                       nchunks, krows, ntiles * 16, kvalid, cout, gw);
-    CHK(hipGetLastError());
-    return LBM_OK;
-}
-
-int bias_grad(lbm_closure* c, const __bf16* dout, int cpad, int cout, float* gb) {
-    constexpr int kBlocks = 512;
-    hipLaunchKernelGGL(bias_partial_kernel, dim3(kBlocks), dim3(256), 0, c->stream, dout, cpad, c->V, c->part);
-    hipLaunchKernelGGL(wgrad_reduce_kernel, dim3(1), dim3(256), 0, c->stream, c->part, kBlocks, 1, cpad, 1, cout,
-                       gb);
+    hipLaunchKernelGGL(wgrad_reduce_kernel, dim3(blocks_for((long)krows * cout, 64)), dim3(256), 0, c->stream,
+                       c->part, blocks, krows, cout, gw, gb);
     CHK(hipGetLastError());
     return LBM_OK;
 }
@@ -536,18 +630,17 @@ int loss_and_grad(lbm_closure* c, const __bf16* x, const float* y, double* loss_
     hipLaunchKernelGGL(loss_grad_kernel, dim3(c->loss_blocks), dim3(256), 0, c->stream, c->pred, y, V, c->d3,
                        c->loss_part);
     // layer 3: weights, bias, input gradient (8-channel padded gradient in, H out)
-    if (int r = wgrad(c, c->a2, H, c->d3, 8, kOutC, c->g + c->off_w[2])) return r;
-    if (int r = bias_grad(c, c->d3, 8, kOutC, c->g + c->off_b[2])) return r;
-    conv<8, 2, 1>(c, io_bf16(c->d3, c->d2), c->pkt[1], nullptr, H, c->stream);
-    hipLaunchKernelGGL(relu_back_kernel, dim3(blocks_for(V * H)), dim3(256), 0, c->stream, c->d2, c->a2, V * H);
+    if (int r = wgrad(c, c->a2, H, c->d3, 8, kOutC, c->g + c->off_w[2], c->g + c->off_b[2])) return r;
+    ConvIO g2 = io_bf16(c->d3, c->d2);
+    g2.act = c->a2;
+    conv<8, 2, 4>(c, g2, c->pkt[1], nullptr, H, c->stream);
     // layer 2
-    if (int r = wgrad(c, c->a1, H, c->d2, H, H, c->g + c->off_w[1])) return r;
-    if (int r = bias_grad(c, c->d2, H, H, c->g + c->off_b[1])) return r;
-    conv<32, 2, 1>(c, io_bf16(c->d2, c->d1), c->pkt[0], nullptr, H, c->stream);
-    hipLaunchKernelGGL(relu_back_kernel, dim3(blocks_for(V * H)), dim3(256), 0, c->stream, c->d1, c->a1, V * H);
+    if (int r = wgrad(c, c->a1, H, c->d2, H, H, c->g + c->off_w[1], c->g + c->off_b[1])) return r;
+    ConvIO g1 = io_bf16(c->d2, c->d1);
+    g1.act = c->a1;
+    conv<32, 2, 4>(c, g1, c->pkt[0], nullptr, H, c->stream);
     // layer 1
-    if (int r = wgrad(c, x, kInC, c->d1, H, H, c->g + c->off_w[0])) return r;
-    if (int r = bias_grad(c, c->d1, H, H, c->g + c->off_b[0])) return r;
+    if (int r = wgrad(c, x, kInC, c->d1, H, H, c->g + c->off_w[0], c->g + c->off_b[0])) return r;
     CHK(hipGetLastError());
     if (loss_out) {
         std::vector<float> lp(c->loss_blocks);
@@ -641,8 +734,7 @@ int lbm_closure_create(int32_t nx, int32_t ny, int32_t nz, int32_t hidden, uint6
         (r = cl_alloc(c, &c->d3, V * 8)) || (r = cl_alloc(c, &c->d2, V * H)) || (r = cl_alloc(c, &c->d1, V * H)) ||
         (r = cl_alloc(c, &c->pred, V * kOutC)))
         return bail(r);
-    const long nchunks = (c->V + c->chunk - 1) / c->chunk;
-    c->part_elems = nchunks * 864L * 32;
+    c->part_elems = (long)c->wg_blocks * (27L * 32 + 1) * 32;
     if ((r = cl_alloc(c, &c->part, c->part_elems))) return bail(r);
     c->loss_blocks = 512;
     if ((r = cl_alloc(c, &c->loss_part, c->loss_blocks))) return bail(r);
