@@ -402,8 +402,25 @@ def run(n_fine=128, samples=48, train_traj=4, heldout=8, heldout_traj=2, spacing
     return report
 
 
+def time_training(n=64, steps=200, samples=4, seed=0):
+    """Device time of the training step alone: a 64^3 network with random
+    (Q, u) samples and targets, `steps` Adam steps timed after 20 untimed
+    ones. Returns ms per step and the forward time."""
+    rng = np.random.default_rng(seed)
+    with ClosureNet(n, n, n, hidden=32, seed=1) as net:
+        for _ in range(samples):
+            net.add_sample(rng.normal(size=(8, n ** 3)), rng.normal(size=(5, n ** 3)) * 0.2)
+        net.train(20, 1e-3)
+        t0 = time.perf_counter()
+        net.train(steps, 1e-3)
+        dt = time.perf_counter() - t0
+        return {"grid": n, "train_ms_per_step": 1e3 * dt / steps, "forward_ms": net.time_forward(50)}
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--time-training", type=int, default=0, metavar="STEPS",
+                    help="only time STEPS training steps on random 64^3 samples")
     ap.add_argument("--fine", type=int, default=128)
     ap.add_argument("--samples", type=int, default=48)
     ap.add_argument("--heldout", type=int, default=8)
@@ -411,6 +428,9 @@ def main():
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--warmup", type=int, default=1500)
     a = ap.parse_args()
+    if a.time_training:
+        print(json.dumps(time_training(steps=a.time_training)), flush=True)
+        return
     rep = run(n_fine=a.fine, samples=a.samples, heldout=a.heldout, iters=a.iters, lr=a.lr, warmup=a.warmup,
               log=lambda s: print(s, flush=True))
     print(json.dumps(rep), flush=True)
