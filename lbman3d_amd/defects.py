@@ -15,10 +15,12 @@ Follows the reference's DefectFinder (src/analysis/defect_finder.tpp):
      (src/sim_io.cc:213-237) writes them as a VTK-HDF UnstructuredGrid of
      VTK_LINE cells (src/analysis/disclination.h:17-61).
 The reference scans its candidate start nodes in std::unordered_map order,
-which depends on the standard library's hashing; here the scan is in
-ascending plaquette id, so the same edges are covered by the same number of
-walks but the order of the lines and the starting node of a closed loop may
-differ from the reference's.
+which depends on the standard library's hash table; here the scan is in
+ascending plaquette id. Both cover every edge exactly once. A line or loop
+without junctions (every plaquette of degree <= 2) becomes the same walk up
+to its direction and, for a loop, its starting plaquette. Where lines meet
+(degree > 2) the split into walks depends on the start order, and the number
+of walks can differ from the reference's.
 """
 from __future__ import annotations
 

@@ -251,3 +251,39 @@ def run_ref(scenario: str, bc: str, mode: str, force_model: int, steps: int, dim
            "F": take(3 * V).reshape(3, V), "f": take(15 * V).reshape(15, V),
            "free_energy": take(1)[0], "log": take(8), "stderr": r.stderr}
     return out
+
+
+def run_ref_defects(bc: str, dims, q, workdir=None):
+    """Run oracle/_ref/ref_defects (the reference's QtensorToOrderDirector and
+    DefectFinder<BC>, built by build_ref.sh for a 14 x 12 x 10 grid) on Q
+    (5 x V) and return the order, director [V][3], the three face arrays and
+    the disclination paths (each an n x 3 array of points)."""
+    import tempfile
+    nx, ny, nz = dims
+    V = nx * ny * nz
+    wd = workdir or tempfile.mkdtemp(prefix="lbmdef_")
+    fin, fout = os.path.join(wd, "in.bin"), os.path.join(wd, "out.bin")
+    with open(fin, "wb") as fp:
+        np.array([nx, ny, nz], dtype=np.int32).tofile(fp)
+        np.ascontiguousarray(q, dtype=np.float64).reshape(-1).tofile(fp)
+    r = subprocess.run([ref_binary("defects"), bc, fin, fout], capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"ref_defects failed: {r.stderr}")
+    raw = open(fout, "rb").read()
+    o = 0
+
+    def take(dtype, n):
+        nonlocal o
+        a = np.frombuffer(raw, dtype=dtype, count=n, offset=o)
+        o += a.nbytes
+        return a.copy()
+    order = take(np.float64, V)
+    director = take(np.float64, 3 * V).reshape(V, 3)
+    faces = tuple(take(np.uint8, n) for n in ((nx * (ny - 1) * (nz - 1)), (nx - 1) * ny * (nz - 1),
+                                              (nx - 1) * (ny - 1) * nz))
+    paths = []
+    for _ in range(int(take(np.int64, 1)[0])):
+        n = int(take(np.int64, 1)[0])
+        paths.append(take(np.float64, 3 * n).reshape(n, 3))
+    assert o == len(raw)
+    return {"order": order, "director": director, "faces": faces, "paths": paths}

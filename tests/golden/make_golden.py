@@ -9,7 +9,11 @@ stepping mode, the input state and the outputs after `steps` steps:
 rho, u, Q, F, f (post-streaming, i*V + cell), the free energy and the
 SimIO::Log sums.
 
-usage: python tests/golden/make_golden.py
+The disclination fixture (defects_ref.npz) holds two Q fields on a
+14 x 12 x 10 grid and, per boundary preset, the reference finder's order,
+director, pierced plaquettes and disclination paths.
+
+usage: python tests/golden/make_golden.py [defects]
 """
 from __future__ import annotations
 
@@ -112,10 +116,64 @@ def make_case(name, scen, bc, mode, fm, steps, init, seed):
     print(f"{name}: {r['stderr'].strip()}")
 
 
+# ---- disclination finder (oracle/_ref/ref_defects: QtensorToOrderDirector and
+# DefectFinder<BC> of the reference, src/analysis_fields.cc and
+# src/analysis/defect_finder.tpp, on a 14 x 12 x 10 grid) ----
+DEFECT_DIMS = (14, 12, 10)
+
+
+def q_of_director(d, S=0.5):
+    """Q = S (n n - I/3) of unit directors d (V x 3), as 5 x V."""
+    return np.stack([S * (d[:, 0] ** 2 - 1 / 3), S * d[:, 0] * d[:, 1], S * d[:, 0] * d[:, 2],
+                     S * (d[:, 1] ** 2 - 1 / 3), S * d[:, 1] * d[:, 2]])
+
+
+def wedge_lines_q(dims, seed):
+    """Two straight wedge lines (+1/2 along z through plaquette centre
+    (3.5, 4.5), -1/2 along x through (y, z) = (7.5, 5.5)), superposed as
+    angles, with a constant out-of-plane tilt (no director lies in the xy
+    plane, where the reference's eigenvector formula degenerates) and a
+    little seeded noise."""
+    nx, ny, nz = dims
+    z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    phi = 0.5 * np.arctan2(y - 4.5, x - 3.5)
+    psi = -0.5 * np.arctan2(z - 5.5, y - 7.5)
+    rng = np.random.default_rng(seed)
+    th = (phi + psi).reshape(-1) + 0.05 * rng.standard_normal(nx * ny * nz)
+    c = np.sqrt(1 - 0.35 ** 2)
+    d = np.stack([c * np.cos(th), c * np.sin(th), np.full_like(th, 0.35)], axis=1)
+    return q_of_director(d)
+
+
+def make_defects():
+    from oracle.pyoracle import run_ref_defects
+    dims = DEFECT_DIMS
+    out = dict(dims=np.array(dims))
+    fields = {"random": seed_director_field(77, *dims, 0.5, 0.01), "lines": wedge_lines_q(dims, 5)}
+    for name, q in fields.items():
+        out[f"{name}_q"] = q
+        for bc in ("FullyPeriodic", "Channel"):
+            r = run_ref_defects(bc, dims, q)
+            key = f"{name}_{bc}"
+            out[f"{key}_order"] = r["order"]
+            out[f"{key}_director"] = r["director"]
+            for a, f in zip("xyz", r["faces"]):
+                out[f"{key}_def_{a}"] = f
+            out[f"{key}_path_len"] = np.array([len(p) for p in r["paths"]], dtype=np.int64)
+            out[f"{key}_path_points"] = (np.concatenate(r["paths"]) if r["paths"] else np.zeros((0, 3)))
+            print(f"defects {key}: {sum(int(f.sum()) for f in r['faces'])} pierced plaquettes, "
+                  f"{len(r['paths'])} paths")
+    np.savez_compressed(os.path.join(OUT, "defects_ref.npz"), **out)
+
+
 def main():
     for k, case in enumerate(CASES):
         make_case(*case, seed=1000 + k)
+    make_defects()
 
 
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["defects"]:
+        make_defects()
+    else:
+        main()
