@@ -44,7 +44,10 @@ BC_BY_NAME = {
 
 
 def golden_names():
-    return sorted(os.path.splitext(os.path.basename(p))[0] for p in glob.glob(os.path.join(GOLDEN_DIR, "*.npz")))
+    """The solver fixtures (the disclination fixture, defects_ref.npz, has
+    its own test)."""
+    names = (os.path.splitext(os.path.basename(p))[0] for p in glob.glob(os.path.join(GOLDEN_DIR, "*.npz")))
+    return sorted(n for n in names if n != "defects_ref")
 
 
 def load_golden(name):
