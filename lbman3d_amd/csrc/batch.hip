@@ -11,10 +11,13 @@
 // consecutive steps differ only in a parity argument and can be captured once
 // as a graph and replayed.
 //
-// Simulations with walls take the split of the single-simulation path: the
+// Simulations with walls in x and y only (z periodic, no anchoring, no
+// moving wall; the slit and channel presets) run the third pipeline's
+// wall-aware form, which computes every cell in the one launch. Other
+// simulations with walls take the split of the single-simulation path: the
 // periodic specialisation over every tile is exact for the cells two or more
 // cells from every open face, and two one-thread-per-cell passes with the
-// full wall handling then overwrite the wall shells of every simulation.
+// full wall handling then overwrite their wall shells.
 #define LBM_FUSED_BODY_ONLY
 #include "step_kernels.hpp"
 #include "fused_step.hpp"
@@ -38,7 +41,8 @@ struct SimDesc {
     int tile_begin;       // first work item of this simulation
     int pull;             // 0 until the first step after an upload
     int aux_always;       // moving walls read rho every step
-    ShellRegion r3, r2;   // shell and halo (Q pass), shell alone (force pass); empty if periodic
+    int walls;            // third pipeline with its wall handling (no shell passes)
+    ShellRegion r3, r2;   // shell and halo (Q pass), shell alone (force pass); empty if periodic or walls
     int sq_begin;         // first item of this simulation in the shell Q pass
     int sf_begin;         // first item of this simulation in the shell force pass
 };
@@ -57,9 +61,11 @@ __device__ __forceinline__ int find_sim(int nsims, int w, Get begin_of) {
 }
 
 // Tile shapes of the batched kernel: kind 0 runs the first fused pipeline
-// over every tile (cells of the wall shells excluded), kind 1 the third
-// pipeline (one barrier per plane; it writes the shells too, and the shell
-// launch overwrites them, as on a single simulation with walls).
+// over every tile (cells of the wall shells excluded, the shell passes write
+// them), kind 1 the third pipeline (one barrier per plane; simulations with
+// walls in x and y only are computed whole by its wall-aware form, others
+// with walls have their shells rewritten by the shell passes, as on a single
+// simulation).
 struct BatchTile {
     int kind, tx, ty, zc;
     const char* name;
@@ -123,12 +129,20 @@ batch_step_kernel_p2(const SimDesc* __restrict__ descs, const BatchItem* __restr
         const SimDesc& d = descs[it.sim];
         const int tx = it.txy % d.ntx, ty = it.txy / d.ntx;
         const int a = aux | d.aux_always;
-        if (d.k.advection != 0)
+        if (d.walls) {
+            if (d.k.advection != 0)
+                p2_range<TX, TY, false, true, true, false, true>(d.f[parity], d.bc, d.k, d.pull, a, tx, ty, it.z0,
+                                                                 it.z1, sIn, sP, sRing);
+            else
+                p2_range<TX, TY, false, false, true, false, true>(d.f[parity], d.bc, d.k, d.pull, a, tx, ty, it.z0,
+                                                                  it.z1, sIn, sP, sRing);
+        } else if (d.k.advection != 0) {
             p2_range<TX, TY, true, true, true>(d.f[parity], d.bc, d.k, d.pull, a, tx, ty, it.z0, it.z1, sIn, sP,
                                                sRing);
-        else
+        } else {
             p2_range<TX, TY, true, false, true>(d.f[parity], d.bc, d.k, d.pull, a, tx, ty, it.z0, it.z1, sIn, sP,
                                                 sRing);
+        }
         __syncthreads();
     }
     if (threadIdx.x == 0) {
@@ -337,8 +351,9 @@ int rebuild(lbm_batch* b) {
         begin += tiles_of(b, (int)i, d.ntx, d.nty, d.nzc);
         d.pull = s->pulled;
         d.aux_always = s->moving ? 1 : 0;
-        d.r3 = make_shell_region(d.bc, 3);
-        d.r2 = make_shell_region(d.bc, 2);
+        d.walls = (!d.periodic && kBatchTiles[b->tile].kind == 1 && p2_walls_ok(d.bc)) ? 1 : 0;
+        d.r3 = d.walls ? ShellRegion{} : make_shell_region(d.bc, 3);
+        d.r2 = d.walls ? ShellRegion{} : make_shell_region(d.bc, 2);
         d.sq_begin = sq_begin;
         d.sf_begin = sf_begin;
         sq_begin += (int)region_cells(d.r3);

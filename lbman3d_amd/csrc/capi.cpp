@@ -156,6 +156,20 @@ int build_slot_table(const lbm_config& c, std::vector<uint16_t>& table) {
     return LBM_OK;
 }
 
+// Whether the pull-slot row of every class a node of this grid can have
+// equals the row of the same x, y class with the z bits cleared.
+bool slots_independent_of_z(const lbm_config& c, const std::vector<uint16_t>& table) {
+    auto bits = [](int i, int n) { return (i == 0 ? 1 : 0) | (i == n - 1 ? 2 : 0); };
+    for (int z : {0, c.nz - 1, std::min(1, c.nz - 1)})
+        for (int y : {0, c.ny - 1, std::min(1, c.ny - 1)})
+            for (int x : {0, c.nx - 1, std::min(1, c.nx - 1)}) {
+                const int cls = bits(x, c.nx) | (bits(y, c.ny) << 2) | (bits(z, c.nz) << 4);
+                for (int j = 0; j < kQ; ++j)
+                    if (table[cls * kQ + j] != table[(cls & 15) * kQ + j]) return false;
+            }
+    return true;
+}
+
 template <typename T>
 BCDesc<T> make_bc(const lbm_config& c) {
     BCDesc<T> b{};
@@ -582,6 +596,7 @@ int lbm_create(const lbm_config* cfg, lbm_handle* out) {
     s->diag_out = s->partials + s->diag_blocks * 10;
     s->bc64 = make_bc<double>(*cfg);
     s->bc32 = make_bc<float>(*cfg);
+    s->bc32.slots_xy = s->bc64.slots_xy = slots_independent_of_z(*cfg, table) ? 1 : 0;
     s->k64 = make_phys<double>(*cfg);
     s->k32 = make_phys<float>(*cfg);
     // FluidFields constructor state (src/fluid_fields.cc:6-17): rho = kDensity,

@@ -71,6 +71,8 @@ struct BCDesc {
     T anchor[6][5];      // strong-anchoring Q per face (xx, xy, xz, yy, yz)
     T wall_u[6][3];      // wall velocity per face (MovingWall; zero otherwise)
     int all_periodic_u;  // every velocity face periodic: no wall slots anywhere
+    int slots_xy;        // every reachable class has the slot row of its x, y
+                         // bits alone (the z bits change nothing)
     T rho_base;          // the density array holds rho - rho_base (rho0 for the
                          // shifted FP32 form, 0 for FP64)
 };

@@ -107,6 +107,34 @@ __device__ __forceinline__ void gather32(const Fields<T>& f, const BCDesc<T>& bc
     }
 }
 
+// Pull gather of the wall-aware third pipeline, given the slot-table row of
+// the cell's x, y class (the rows do not depend on z: BCDesc::slots_xy)
+// packed two 16-bit entries per word. No face has a moving wall, so a
+// bounce-back slot reads its source population at the cell itself like a
+// mirror slot, and each population is one load from a selected (population,
+// cell) pair: the values of gather32<false> for those tags.
+__device__ __forceinline__ void gather32_row(const Fields<float>& f, const BCDesc<float>& bc, int x, int y, int z,
+                                             unsigned c, int pull, const uint32_t* srow, float out[kQ]) {
+    const int nx = bc.n[0], ny = bc.n[1], nz = bc.n[2];
+    const unsigned vs4 = (unsigned)f.vs * 4u;
+    if (!pull) {
+#pragma unroll
+        for (int j = 0; j < kQ; ++j) out[j] = ldo(f.g, j * vs4 + c * 4u);
+        return;
+    }
+    const int xm = x == 0 ? nx - 1 : x - 1, xp = x == nx - 1 ? 0 : x + 1;
+    const int ym = y == 0 ? ny - 1 : y - 1, yp = y == ny - 1 ? 0 : y + 1;
+    const int zm = z == 0 ? nz - 1 : z - 1, zp = z == nz - 1 ? 0 : z + 1;
+    const int xs[3] = {xp, x, xm}, ys[3] = {yp, y, ym}, zs[3] = {zp, z, zm};
+#pragma unroll
+    for (int j = 0; j < kQ; ++j) {
+        const uint16_t e = (uint16_t)(srow[j >> 1] >> (16 * (j & 1)));
+        const unsigned nb = ((unsigned)zs[cz(j) + 1] * ny + ys[cy(j) + 1]) * nx + xs[cx(j) + 1];
+        const unsigned off = slot_kind(e) == SLOT_STREAM ? j * vs4 + nb * 4u : slot_src(e) * vs4 + c * 4u;
+        out[j] = ldo(f.g, off);
+    }
+}
+
 template <int TX, int TY>
 struct TileGeom {
     static constexpr int NT = TX * TY;               // threads = interior columns
@@ -186,7 +214,9 @@ __device__ __forceinline__ void stage_a(const BCDesc<T>& bc, const Phys<T>& k,
 // Operand order inside each formula follows q_pointwise; the Ericksen force
 // is accumulated per component (a different summation order, within FP32
 // rounding of q_pointwise).
-template <int TX, int TY, bool NOGHOST, bool UPWIND>
+// GH: 1 every face periodic (no ghosts), 0 the general ghost rules, 2 walls
+// on x and y faces only (Neumann Q; no-slip or specular velocity; z periodic).
+template <int TX, int TY, int GH, bool UPWIND>
 __device__ __forceinline__ void stage_a_lean(const BCDesc<float>& bc, const Phys<float>& k,
                                              const float (*sIn)[TileGeom<TX, TY>::NI], int ii, int jj,
                                              const float zm[kNumIn], const float c[kNumIn],
@@ -200,9 +230,15 @@ __device__ __forceinline__ void stage_a_lean(const BCDesc<float>& bc, const Phys
 #pragma unroll
     for (int A = 0; A < 3; ++A) {
         const float v0 = c[5 + A];
-        g[A][0] = u_grad<float, NOGHOST>(bc, 0, A, upos[0], sIn[5 + A][xm], v0, sIn[5 + A][xp]);
-        g[A][1] = u_grad<float, NOGHOST>(bc, 1, A, upos[1], sIn[5 + A][ym], v0, sIn[5 + A][yp]);
-        g[A][2] = u_grad<float, NOGHOST>(bc, 2, A, upos[2], zm[5 + A], v0, zp[5 + A]);
+        if constexpr (GH == 2) {
+            g[A][0] = u_grad_xy<float>(bc, 0, A, upos[0], sIn[5 + A][xm], v0, sIn[5 + A][xp]);
+            g[A][1] = u_grad_xy<float>(bc, 1, A, upos[1], sIn[5 + A][ym], v0, sIn[5 + A][yp]);
+            g[A][2] = u_grad<float, true>(bc, 2, A, upos[2], zm[5 + A], v0, zp[5 + A]);
+        } else {
+            g[A][0] = u_grad<float, GH == 1>(bc, 0, A, upos[0], sIn[5 + A][xm], v0, sIn[5 + A][xp]);
+            g[A][1] = u_grad<float, GH == 1>(bc, 1, A, upos[1], sIn[5 + A][ym], v0, sIn[5 + A][yp]);
+            g[A][2] = u_grad<float, GH == 1>(bc, 2, A, upos[2], zm[5 + A], v0, zp[5 + A]);
+        }
     }
     const float Exx = g[0][0], Eyy = g[1][1];
     const float Exy = 0.5f * (g[0][1] + g[1][0]);
@@ -222,7 +258,10 @@ __device__ __forceinline__ void stage_a_lean(const BCDesc<float>& bc, const Phys
     float dxx0 = 0.0f, dxx1 = 0.0f, dxx2 = 0.0f;   // grad Qxx, kept for the cross term with Hyy
 #pragma unroll
     for (int m = 0; m < 5; ++m) {
-        const Deriv<float> d = q_deriv<float, NOGHOST>(bc, m, qpos, c[m], sIn[m][xm], sIn[m][xp], sIn[m][ym], sIn[m][yp], zm[m], zp[m]);
+        const Deriv<float> d =
+            GH == 2 ? q_deriv_xy<float>(bc, qpos, c[m], sIn[m][xm], sIn[m][xp], sIn[m][ym], sIn[m][yp], zm[m], zp[m])
+                    : q_deriv<float, GH == 1>(bc, m, qpos, c[m], sIn[m][xm], sIn[m][xp], sIn[m][ym], sIn[m][yp], zm[m],
+                                               zp[m]);
         float Q2;
         switch (m) {
             case 0: Q2 = Qxx * Qxx + Qxy * Qxy + Qxz * Qxz - third * TrQ2; break;
@@ -928,6 +967,24 @@ __device__ __forceinline__ void p_sets(float alpha, const StageA& a, float pxy[6
     pz[2] = alpha * (q[0] + q[3]) - (sg[0] + sg[3]);
 }
 
+// Boundary tags the third pipeline's WALLS mode computes exactly: z faces
+// periodic (and a slot table whose rows do not depend on the z position:
+// BCDesc::slots_xy), x and y faces with a periodic or Neumann Q tag and a periodic,
+// no-slip or specular velocity tag, periodic for both or for neither
+// (anchoring changes the Q ghost of the force stage, a moving wall's slot
+// reads the density, which this pipeline stores only with aux, and equal
+// periodicity lets Q and velocity share the mapped columns).
+inline bool p2_walls_ok(const BCDesc<float>& bc) {
+    if (!bc.slots_xy) return false;
+    for (int face = 4; face < 6; ++face)
+        if (bc.qkind[face] != QB_PERIODIC || bc.ukind[face] != UB_PERIODIC) return false;
+    for (int face = 0; face < 4; ++face) {
+        if (bc.qkind[face] == QB_ANCHOR || bc.ukind[face] == UB_MOVING) return false;
+        if ((bc.qkind[face] == QB_PERIODIC) != (bc.ukind[face] == UB_PERIODIC)) return false;
+    }
+    return true;
+}
+
 // Third pipeline (periodic specialisation, like the second): the force
 // stage reads only the divergence-form sets above (6 values per column in
 // LDS instead of 13; the z set of the own column stays in registers), and the
@@ -942,7 +999,17 @@ __device__ __forceinline__ void p_sets(float alpha, const StageA& a, float pxy[6
 // One call computes z planes [z0, z1) (0 <= z0 < z1 <= nz) of the TX x TY
 // column tile (tx, ty), with LDS buffers from the calling kernel; a kernel
 // that runs several tiles per block puts a barrier between them.
-template <int TX, int TY, bool MASKED, bool UPWIND, bool NT, bool UNCOND = false>
+//
+// WALLS: the x and y faces may be walls (Q tag periodic or Neumann, velocity
+// tag periodic, no-slip or specular; the z faces periodic, which the caller
+// checks with p2_walls_ok). Every cell is then computed exactly, with the
+// unfused kernels' wall handling: input columns mapped by the face tags,
+// ghost values in the Q update, the slot table in the pull gather, and the
+// force stage's x and y neighbours clamped to the cell itself where the
+// reference clamps. For a Neumann face the clamped neighbour is the ghost of
+// Q^{n+1} as well as of Sigma and Tau, so the divergence-form sets clamp as
+// a whole.
+template <int TX, int TY, bool MASKED, bool UPWIND, bool NT, bool UNCOND = false, bool WALLS = false>
 __device__ __forceinline__ void p2_range(const Fields<float>& f, const BCDesc<float>& bc, const Phys<float>& k,
                                         int pull, int aux, int tx, int ty, int z0, int z1,
                                         float (*sIn)[kNumIn][TileGeom<TX, TY>::NI],
@@ -966,10 +1033,20 @@ __device__ __forceinline__ void p2_range(const Fields<float>& f, const BCDesc<fl
     int oi = 0, oj = 0;
     if (has_outer) ring_col(orank, G::IX, G::IY, oi, oj);
 
-    const int qx0 = map_col(gx, nx, true, true), qy0 = map_col(gy, ny, true, true);
-    const int qxr = map_col(x0 - 1 + ra, nx, true, true), qyr = map_col(y0 - 1 + rb, ny, true, true);
+    // Periodic flags of the x and y faces (all periodic unless WALLS; Q and
+    // velocity agree, see p2_walls_ok).
+    bool plo[2] = {true, true}, phi[2] = {true, true};
+    if constexpr (WALLS) {
+#pragma unroll
+        for (int a = 0; a < 2; ++a) {
+            plo[a] = bc.qkind[2 * a] == QB_PERIODIC;
+            phi[a] = bc.qkind[2 * a + 1] == QB_PERIODIC;
+        }
+    }
+    const int qx0 = map_col(gx, nx, plo[0], phi[0]), qy0 = map_col(gy, ny, plo[1], phi[1]);
+    const int qxr = map_col(x0 - 1 + ra, nx, plo[0], phi[0]), qyr = map_col(y0 - 1 + rb, ny, plo[1], phi[1]);
     int oc = 0;
-    if (has_outer) oc = map_col(y0 - 2 + oj, ny, true, true) * nx + map_col(x0 - 2 + oi, nx, true, true);
+    if (has_outer) oc = map_col(y0 - 2 + oj, ny, plo[1], phi[1]) * nx + map_col(x0 - 2 + oi, nx, plo[0], phi[0]);
     const int c0 = qy0 * nx + qx0, cr = qyr * nx + qxr;
     auto zplane = [&](int z) { return (unsigned)map_coord(z, nz, true, true) * plane; };
     const int cc_int = (ib + 1) * G::IX + ia + 1;
@@ -977,6 +1054,35 @@ __device__ __forceinline__ void p2_range(const Fields<float>& f, const BCDesc<fl
     const int cc_out = oj * G::IX + oi;
     const int rcell = rb * G::AX + ra;
     const int qpos_i[3] = {qx0, qy0, 0}, qpos_r[3] = {qxr, qyr, 0};
+    // Force-stage x and y neighbours in the set arrays: the cell itself where
+    // a face clamps (Neumann).
+    int lxm = acell - 1, lxp = acell + 1, lym = acell - G::AX, lyp = acell + G::AX;
+    // Slot-table rows of the 16 x, y classes (no z bits; see
+    // BCDesc::slots_xy), two entries per word, in LDS; the class of this
+    // column; and
+    // whether any column of the wavefront lies on an x or y face (else the
+    // periodic gather serves the whole wavefront). The first pull gather
+    // follows two barriers, so the rows are in place by then; a caller that
+    // runs several ranges per block puts a barrier between them.
+    int cxy = 0;
+    bool wave_walls = false;
+    const uint32_t* slot_rows = nullptr;
+    if constexpr (WALLS) {
+        if (qx0 == 0 && !plo[0]) lxm = acell;
+        if (qx0 == nx - 1 && !phi[0]) lxp = acell;
+        if (qy0 == 0 && !plo[1]) lym = acell;
+        if (qy0 == ny - 1 && !phi[1]) lyp = acell;
+        __shared__ uint32_t sSlot[16][8];
+        if (t < 16 * 8) {
+            const int cl = t >> 3, w = t & 7;
+            const uint16_t* row = f.slots + cl * kQ;
+            const uint32_t lo = row[2 * w], hi = 2 * w + 1 < kQ ? row[2 * w + 1] : 0u;
+            sSlot[cl][w] = lo | (hi << 16);
+        }
+        cxy = node_class(min(gx, nx - 1), min(gy, ny - 1), 1, nx, ny, 3);
+        wave_walls = __builtin_amdgcn_ballot_w64(cxy != 0) != 0;
+        slot_rows = &sSlot[0][0];
+    }
 
     // Prologue: interior planes z0-2 .. z0 (registers), ring planes z0-2 and
     // z0-1 (LDS) and z0 (registers, stored in the first iteration), outer
@@ -1030,7 +1136,14 @@ __device__ __forceinline__ void p2_range(const Fields<float>& f, const BCDesc<fl
             if (has_ring) load_column<TX, TY>(f, cr, cr, zplane(pn), zplane(pn), rnext);
             if (has_outer) load_column<TX, TY>(f, oc, oc, zplane(pa + 1), zplane(pa + 1), onext);
         }
-        if (do_b) gather32<true>(f, bc, gx, gy, pb, cb, pull, pop);
+        if (do_b) {
+            if constexpr (WALLS) {
+                if (wave_walls) gather32_row(f, bc, gx, gy, pb, cb, pull, slot_rows + 8 * cxy, pop);
+                else gather32<true>(f, bc, gx, gy, pb, cb, pull, pop);
+            } else {
+                gather32<true>(f, bc, gx, gy, pb, cb, pull, pop);
+            }
+        }
 #pragma unroll
         for (int m = 0; m < 6; ++m) sP[bi ^ 1][m][acell] = pxy_prev[m];
 #pragma unroll
@@ -1056,8 +1169,8 @@ __device__ __forceinline__ void p2_range(const Fields<float>& f, const BCDesc<fl
         float fe_new[3], z_new[3], pxy_new[6];
         {
             const int qpos[3] = {qpos_i[0], qpos_i[1], zq};
-            stage_a_lean<TX, TY, true, UPWIND>(bc, k, sIn[bi], ia + 1, ib + 1, hin[0], hin[1], hin[2], qpos, qpos, ai,
-                                               fe_new);
+            stage_a_lean<TX, TY, WALLS ? 2 : 1, UPWIND>(bc, k, sIn[bi], ia + 1, ib + 1, hin[0], hin[1], hin[2], qpos,
+                                                        qpos, ai, fe_new);
             p_sets(k.alpha, ai, pxy_new, z_new);
         }
         float pr_new[6] = {0, 0, 0, 0, 0, 0};
@@ -1071,7 +1184,7 @@ __device__ __forceinline__ void p2_range(const Fields<float>& f, const BCDesc<fl
                 zp[m] = sRing[(pa + 1) & 3][m][t];
             }
             StageA ar;
-            stage_a_lean<TX, TY, true, UPWIND>(bc, k, sIn[bi], ra + 1, rb + 1, zm, c, zp, qpos, qpos, ar, fer);
+            stage_a_lean<TX, TY, WALLS ? 2 : 1, UPWIND>(bc, k, sIn[bi], ra + 1, rb + 1, zm, c, zp, qpos, qpos, ar, fer);
             p_sets(k.alpha, ar, pr_new, zr);
         }
 
@@ -1082,8 +1195,8 @@ __device__ __forceinline__ void p2_range(const Fields<float>& f, const BCDesc<fl
             if (k.force_full) {
 #pragma unroll
                 for (int a = 0; a < 3; ++a) {
-                    const float dx = P[a * G::NA + acell + 1] - P[a * G::NA + acell - 1];
-                    const float dy = P[(3 + a) * G::NA + acell + G::AX] - P[(3 + a) * G::NA + acell - G::AX];
+                    const float dx = P[a * G::NA + lxp] - P[a * G::NA + lxm];
+                    const float dy = P[(3 + a) * G::NA + lyp] - P[(3 + a) * G::NA + lym];
                     const float dz = z_new[a] - zm2[a];
                     F[a] = fe_prev[a] + (0.5f * ((dx + dy) + dz) - k.mu * hin[0][5 + a]);
                 }
@@ -1143,18 +1256,19 @@ __device__ __forceinline__ void p2_range(const Fields<float>& f, const BCDesc<fl
 }
 
 // Chunk tz of ZC planes of the column tile (tx, ty).
-template <int TX, int TY, int ZC, bool MASKED, bool UPWIND, bool NT, bool UNCOND = false>
+template <int TX, int TY, int ZC, bool MASKED, bool UPWIND, bool NT, bool UNCOND = false, bool WALLS = false>
 __device__ __forceinline__ void p2_tile(const Fields<float>& f, const BCDesc<float>& bc, const Phys<float>& k,
                                        int pull, int aux, int tx, int ty, int tz,
                                        float (*sIn)[kNumIn][TileGeom<TX, TY>::NI],
                                        float (*sP)[6][TileGeom<TX, TY>::NA],
                                        float (*sRing)[kNumIn][TileGeom<TX, TY>::NR]) {
     const int z0 = tz * ZC;
-    p2_range<TX, TY, MASKED, UPWIND, NT, UNCOND>(f, bc, k, pull, aux, tx, ty, z0, min(z0 + ZC, bc.n[2]), sIn, sP,
-                                                sRing);
+    p2_range<TX, TY, MASKED, UPWIND, NT, UNCOND, WALLS>(f, bc, k, pull, aux, tx, ty, z0, min(z0 + ZC, bc.n[2]), sIn,
+                                                       sP, sRing);
 }
 
-template <int TX, int TY, int ZC, bool MASKED, int WPE, bool UPWIND, bool NT, bool ZSLOW, bool UNCOND = false>
+template <int TX, int TY, int ZC, bool MASKED, int WPE, bool UPWIND, bool NT, bool ZSLOW, bool UNCOND = false,
+          bool WALLS = false>
 __global__ void __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(WPE, WPE)))
 fused_step_kernel_p2(Fields<float> f, BCDesc<float> bc, Phys<float> k, int pull, int aux, int ntx, int nty,
                      int nzc) {
@@ -1178,7 +1292,20 @@ fused_step_kernel_p2(Fields<float> f, BCDesc<float> bc, Phys<float> k, int pull,
     } else {
         tz = L % nzc; tx = (L / nzc) % ntx; ty = L / (nzc * ntx);
     }
-    p2_tile<TX, TY, ZC, MASKED, UPWIND, NT, UNCOND>(f, bc, k, pull, aux, tx, ty, tz, sIn, sP, sRing);
+    if constexpr (WALLS) {
+        // A tile with a cell within two cells of a wall runs the wall-aware
+        // form. The others are whole tiles of cells whose inputs, halo and
+        // pull sources lie inside the domain away from every wall, which the
+        // periodic form computes exactly (with UNCOND, every memory
+        // instruction unconditional).
+        const int x0 = tx * TX, y0 = ty * TY, nx = bc.n[0], ny = bc.n[1];
+        const bool near = (bc.qkind[0] != QB_PERIODIC && x0 < 2) || (bc.qkind[1] != QB_PERIODIC && x0 + TX > nx - 2) ||
+                          (bc.qkind[2] != QB_PERIODIC && y0 < 2) || (bc.qkind[3] != QB_PERIODIC && y0 + TY > ny - 2);
+        if (near) p2_tile<TX, TY, ZC, false, UPWIND, NT, false, true>(f, bc, k, pull, aux, tx, ty, tz, sIn, sP, sRing);
+        else p2_tile<TX, TY, ZC, false, UPWIND, NT, UNCOND, false>(f, bc, k, pull, aux, tx, ty, tz, sIn, sP, sRing);
+    } else {
+        p2_tile<TX, TY, ZC, MASKED, UPWIND, NT, UNCOND, false>(f, bc, k, pull, aux, tx, ty, tz, sIn, sP, sRing);
+    }
 }
 
 #ifndef LBM_FUSED_BODY_ONLY
@@ -1283,19 +1410,26 @@ static void launch_tile(const Fields<float>& f, const BCDesc<float>& bc, const P
     } else {
         // kinds 5, 6: every memory instruction unconditional; only for
         // periodic grids whose x and y sides are multiples of the tile
-        // (otherwise the conditional form of the same kind runs).
+        // (otherwise the conditional form of the same kind runs), and for
+        // the tiles away from the walls of a domain with walls in x and y.
         constexpr bool uc = KIND >= 5;
         constexpr bool zs = KIND == 4 || KIND == 6;
         const bool full = periodic && bc.n[0] % TX == 0 && bc.n[1] % TY == 0;
-#define LBM_FUSED_P2(M, U, UC)                                                                                     \
-    hipLaunchKernelGGL((fused_step_kernel_p2<TX, TY, ZC, M, 2, U, true, zs, UC>), dim3(grid), dim3(TX * TY), 0, s, f, \
-                       bc, k, pull, aux, ntx, nty, nzc)
+#define LBM_FUSED_P2(M, U, UC, W)                                                                                  \
+    hipLaunchKernelGGL((fused_step_kernel_p2<TX, TY, ZC, M, 2, U, true, zs, UC, W>), dim3(grid), dim3(TX * TY), 0, s, \
+                       f, bc, k, pull, aux, ntx, nty, nzc)
         if (uc && full) {
-            if (up) LBM_FUSED_P2(false, true, uc); else LBM_FUSED_P2(false, false, uc);
+            if (up) LBM_FUSED_P2(false, true, uc, false); else LBM_FUSED_P2(false, false, uc, false);
         } else if (periodic) {
-            if (up) LBM_FUSED_P2(false, true, false); else LBM_FUSED_P2(false, false, false);
+            if (up) LBM_FUSED_P2(false, true, false, false); else LBM_FUSED_P2(false, false, false, false);
+        } else if (p2_walls_ok(bc)) {
+            // walls in x and y only: every cell exact in the one launch (tiles
+            // away from the walls on the periodic form, unconditional in the
+            // kinds that are)
+            if (up) LBM_FUSED_P2(false, true, uc, true); else LBM_FUSED_P2(false, false, uc, true);
+            return;
         } else {
-            if (up) LBM_FUSED_P2(true, true, false); else LBM_FUSED_P2(true, false, false);
+            if (up) LBM_FUSED_P2(true, true, false, false); else LBM_FUSED_P2(true, false, false, false);
         }
 #undef LBM_FUSED_P2
     }

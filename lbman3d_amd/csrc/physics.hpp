@@ -85,6 +85,45 @@ __device__ __forceinline__ T u_grad(const BCDesc<T>& bc, int a, int comp, int i,
     return (p - m) / T(2.0);
 }
 
+// The same derivatives for domains whose x and y faces are periodic or walls
+// with a Neumann Q tag and a periodic, no-slip or specular velocity tag, and
+// whose z faces are periodic (the fused step's wall-aware form): q_pair and
+// u_ghost reduced to those tags, with the z axis taken as periodic.
+template <typename T>
+__device__ __forceinline__ void q_pair_neumann(const BCDesc<T>& bc, int a, int i, T qm, T q0, T qp, T& m, T& p) {
+    const bool ghost = bc.qkind[2 * a] != QB_PERIODIC;
+    m = (ghost && i == 0) ? q0 : qm;
+    p = (ghost && i == bc.n[a] - 1) ? q0 : qp;
+}
+template <typename T>
+__device__ __forceinline__ Deriv<T> q_deriv_xy(const BCDesc<T>& bc, const int pos[3], T q0, T xm, T xp, T ym, T yp,
+                                               T zm, T zp) {
+    T mx, px, my, py;
+    q_pair_neumann(bc, 0, pos[0], xm, q0, xp, mx, px);
+    q_pair_neumann(bc, 1, pos[1], ym, q0, yp, my, py);
+    Deriv<T> r;
+    r.d[0] = (px - mx) / T(2.0);
+    r.d[1] = (py - my) / T(2.0);
+    r.d[2] = (zp - zm) / T(2.0);
+    r.lap = (mx + px + my + py + zm + zp) - T(6.0) * q0;
+    r.d2[0] = (mx + px) - T(2.0) * q0;
+    r.d2[1] = (my + py) - T(2.0) * q0;
+    r.d2[2] = (zm + zp) - T(2.0) * q0;
+    return r;
+}
+// Velocity gradient along x or y: no-slip ghost -v, specular ghost -v for
+// the normal component and v for the tangential ones.
+template <typename T>
+__device__ __forceinline__ T u_grad_xy(const BCDesc<T>& bc, int a, int comp, int i, T vm, T v0, T vp) {
+    T m = vm, p = vp;
+    if (bc.ukind[2 * a] != UB_PERIODIC) {
+        const bool tangential = comp != a;
+        if (i == 0) m = (bc.ukind[2 * a] == UB_SPECULAR && tangential) ? v0 : -v0;
+        if (i == bc.n[a] - 1) p = (bc.ukind[2 * a + 1] == UB_SPECULAR && tangential) ? v0 : -v0;
+    }
+    return (p - m) / T(2.0);
+}
+
 // One axis of u.grad(Q) (AdvectiveAxisTerm, src/model.h:60-68).
 template <typename T>
 __device__ __forceinline__ T adv_axis(int upwind, T u, T dq, T d2q) {
