@@ -1054,6 +1054,18 @@ __device__ __forceinline__ void p2_range(const Fields<float>& f, const BCDesc<fl
     const int cc_out = oj * G::IX + oi;
     const int rcell = rb * G::AX + ra;
     const int qpos_i[3] = {qx0, qy0, 0}, qpos_r[3] = {qxr, qyr, 0};
+    // In-plane parts of the periodic pull sources (x - cx, y - cy) of this
+    // column, [cy + 1][cx + 1]: each plane's gather adds the plane offset of
+    // z - cz, which is uniform over the block.
+    unsigned ioff[3][3];
+    {
+        const int xs[3] = {qx0 == nx - 1 ? 0 : qx0 + 1, qx0, qx0 == 0 ? nx - 1 : qx0 - 1};
+        const int ys[3] = {qy0 == ny - 1 ? 0 : qy0 + 1, qy0, qy0 == 0 ? ny - 1 : qy0 - 1};
+#pragma unroll
+        for (int b = 0; b < 3; ++b)
+#pragma unroll
+            for (int a = 0; a < 3; ++a) ioff[b][a] = (unsigned)ys[b] * nx + xs[a];
+    }
     // Force-stage x and y neighbours in the set arrays: the cell itself where
     // a face clamps (Neumann).
     int lxm = acell - 1, lxp = acell + 1, lym = acell - G::AX, lyp = acell + G::AX;
@@ -1147,7 +1159,16 @@ __device__ __forceinline__ void p2_range(const Fields<float>& f, const BCDesc<fl
                 if (wave_walls) gather32_row(f, bc, gx, gy, pb, cb, pull, slot_rows + 8 * cxy, pop);
                 else gather32<true>(f, bc, gx, gy, pb, cb, pull, pop);
             } else {
-                gather32<true>(f, bc, gx, gy, pb, cb, pull, pop);
+                const unsigned vs4 = (unsigned)f.vs * 4u;
+                if (pull) {
+                    const unsigned zoff[3] = {zplane(pb + 1), zplane(pb), zplane(pb - 1)};
+#pragma unroll
+                    for (int j = 0; j < kQ; ++j)
+                        pop[j] = ldo(f.g, j * vs4 + (zoff[cz(j) + 1] + ioff[cy(j) + 1][cx(j) + 1]) * 4u);
+                } else {
+#pragma unroll
+                    for (int j = 0; j < kQ; ++j) pop[j] = ldo(f.g, j * vs4 + cb * 4u);
+                }
             }
         }
         __builtin_amdgcn_s_setprio(0);
