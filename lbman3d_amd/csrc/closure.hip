@@ -151,6 +151,36 @@ conv_fwd_kernel(ConvIO io, const bf16x8* __restrict__ wpk, const float* __restri
     }
     // D fragment: lane holds rows 4*kb + r (x within the row tile), column n*16 + m (channel).
     const long V = (long)nx * ny * nz;
+    if constexpr (EPI == 2 || EPI == 3) {
+        // FP32 channels-first output: in fragment order each store would
+        // touch 4 cells 16 B apart per channel. The values go through LDS
+        // (the input tile is no longer read) so that a thread finishes one
+        // cell for every channel and a wavefront's accesses to one channel
+        // array are 64 consecutive cells.
+        static_assert(CIN == 32 && NT == 1, "final layer: 32 input channels, at most 16 outputs");
+        constexpr int NC = WZ * kTY * kTX;
+        static_assert(16 * NC * sizeof(float) <= sizeof(tile), "staging fits the input tile");
+        float* so = reinterpret_cast<float*>(tile);
+        __syncthreads();
+        if (m < cout) {
+            const float bco = bias[m];
+#pragma unroll
+            for (int t = 0; t < kTY; ++t)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) so[m * NC + (w * kTY + t) * kTX + 4 * kb + r] = acc[t][0][r] + bco;
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < NC; i += 64 * WZ) {
+            const int lx = i % kTX, ly = (i / kTX) % kTY, lz = i / (kTX * kTY);
+            const long cell = ((long)(bz * WZ + lz) * ny + by * kTY + ly) * nx + bx * kTX + lx;
+            for (int co = 0; co < cout; ++co) {
+                const float v = so[co * NC + i];
+                if (EPI == 3) io.q[co][cell] += io.alpha * v / io.out_scale[co];
+                else reinterpret_cast<float*>(io.out)[(long)co * V + cell] = v;
+            }
+        }
+        return;
+    }
 #pragma unroll
     for (int n = 0; n < NT; ++n) {
         const int co = n * 16 + m;
