@@ -466,10 +466,12 @@ __device__ __forceinline__ void fused_body(const Fields<T>& f, const BCDesc<T>& 
     auto qplane = [&](int z) { return (unsigned)map_coord(z, nz, ax[2].qlo, ax[2].qhi) * plane; };
     auto uplane = [&](int z) { return (unsigned)map_coord(z, nz, ax[2].ulo, ax[2].uhi) * plane; };
 
-    // Input history of the interior column: planes pa-1, pa, pa+1 (registers).
-    T hin[3][kNumIn];
+    // Input history of the interior column: planes pa-1, pa, pa+1 (registers),
+    // and the next plane, pa + 2, loaded one iteration ahead of its use.
+    T hin[3][kNumIn], hnext[kNumIn];
     load_column<TX, TY>(f, qc0, uc0, qplane(z0 - 2), uplane(z0 - 2), hin[1]);
     load_column<TX, TY>(f, qc0, uc0, qplane(z0 - 1), uplane(z0 - 1), hin[2]);
+    load_column<TX, TY>(f, qc0, uc0, qplane(z0), uplane(z0), hnext);
     if (has_ring) {
         T v[kNumIn];
         load_column<TX, TY>(f, qcr, ucr, qplane(z0 - 2), uplane(z0 - 2), v);
@@ -506,8 +508,12 @@ __device__ __forceinline__ void fused_body(const Fields<T>& f, const BCDesc<T>& 
         for (int m = 0; m < kNumIn; ++m) {
             hin[0][m] = hin[1][m];
             hin[1][m] = hin[2][m];
+            hin[2][m] = hnext[m];
         }
-        load_column<TX, TY>(f, qc0, uc0, qplane(pa + 1), uplane(pa + 1), hin[2]);
+        // At pa == z1 the loaded plane is never used; clamping it to z1 + 1
+        // keeps every request within one plane past the chunk.
+        const int pn = min(pa + 2, z1 + 1);
+        load_column<TX, TY>(f, qc0, uc0, qplane(pn), uplane(pn), hnext);
         // The pull gather of plane pb = pa - 1 reads the previous state only,
         // so it is issued here and completes under the barrier and the Q
         // update. (The moving-wall term reads rho of the own cell, which only
