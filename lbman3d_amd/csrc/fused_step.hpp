@@ -408,7 +408,16 @@ __device__ __forceinline__ CellMask cell_mask(const BCDesc<T>& bc, int mode, int
 // column tile (tx, ty), writing the cells that `mode` selects. LDS buffers
 // come from the calling kernel so that a kernel instantiating several
 // variants does not allocate LDS per variant.
-template <int TX, int TY, int ZC, bool PERIODIC, typename T>
+//
+// UNCOND (fully periodic grids whose x and y sides are whole tiles, every
+// cell written): the ring and outer-ring loads, the pull gather and the
+// stores are issued by every thread and the first two planes of the chunk
+// are peeled, so that no memory instruction sits under a branch. The memory
+// counter waits the compiler places after such a branch assume its loads or
+// stores were skipped and then wait for everything issued before, the
+// previous plane's stores included, and at one wave per SIMD (FP64) nothing
+// covers that wait.
+template <int TX, int TY, int ZC, bool PERIODIC, typename T, bool UNCOND = false>
 __device__ __forceinline__ void fused_body(const Fields<T>& f, const BCDesc<T>& bc, const Phys<T>& k,
                                            int pull, int aux, int tx, int ty, int tz,
                                            T (*sIn)[TileGeom<TX, TY>::NI],
@@ -466,12 +475,17 @@ __device__ __forceinline__ void fused_body(const Fields<T>& f, const BCDesc<T>& 
     auto qplane = [&](int z) { return (unsigned)map_coord(z, nz, ax[2].qlo, ax[2].qhi) * plane; };
     auto uplane = [&](int z) { return (unsigned)map_coord(z, nz, ax[2].ulo, ax[2].uhi) * plane; };
 
-    // Input history of the interior column: planes pa-1, pa, pa+1 (registers),
-    // and the next plane, pa + 2, loaded one iteration ahead of its use.
+    // Input history of the interior column in iteration pa: planes pa-1, pa,
+    // pa+1 (registers); the next plane, pa + 2, is loaded one iteration ahead
+    // of its use. It joins the history at the end of the iteration in the
+    // unconditional form (whose waits then fall after the stores are
+    // issued), at the start of the next one otherwise (measured faster
+    // there on the FP32 tiles).
+    static_assert(!UNCOND || PERIODIC, "unconditional form: fully periodic grids");
     T hin[3][kNumIn], hnext[kNumIn];
-    load_column<TX, TY>(f, qc0, uc0, qplane(z0 - 2), uplane(z0 - 2), hin[1]);
-    load_column<TX, TY>(f, qc0, uc0, qplane(z0 - 1), uplane(z0 - 1), hin[2]);
-    load_column<TX, TY>(f, qc0, uc0, qplane(z0), uplane(z0), hnext);
+    load_column<TX, TY>(f, qc0, uc0, qplane(z0 - 2), uplane(z0 - 2), hin[UNCOND ? 0 : 1]);
+    load_column<TX, TY>(f, qc0, uc0, qplane(z0 - 1), uplane(z0 - 1), hin[UNCOND ? 1 : 2]);
+    load_column<TX, TY>(f, qc0, uc0, qplane(z0), uplane(z0), UNCOND ? hin[2] : hnext);
     if (has_ring) {
         T v[kNumIn];
         load_column<TX, TY>(f, qcr, ucr, qplane(z0 - 2), uplane(z0 - 2), v);
@@ -484,15 +498,19 @@ __device__ __forceinline__ void fused_body(const Fields<T>& f, const BCDesc<T>& 
     // Ring and outer-ring inputs are loaded one plane ahead of their LDS
     // store: ring plane pa + 1 and outer plane pa arrive in iteration pa.
     T rnext[kNumIn], onext[kNumIn];
-    if (has_ring) load_column<TX, TY>(f, qcr, ucr, qplane(z0), uplane(z0), rnext);
+    // Columns of the ring and outer-ring loads; in the unconditional form a
+    // thread without such a column loads its own (the values go unused).
+    const int rq = (UNCOND && !has_ring) ? qc0 : qcr, ru = (UNCOND && !has_ring) ? uc0 : ucr;
+    const int oq = (UNCOND && !has_outer) ? qc0 : oqc, ou = (UNCOND && !has_outer) ? uc0 : ouc;
+    if (UNCOND || has_ring) load_column<TX, TY>(f, rq, ru, qplane(z0), uplane(z0), rnext);
     auto load_outer = [&](int z, T* v) {
-        const unsigned qo = (qplane(z) + (unsigned)oqc) * E, uo = (uplane(z) + (unsigned)ouc) * E;
+        const unsigned qo = (qplane(z) + (unsigned)oq) * E, uo = (uplane(z) + (unsigned)ou) * E;
 #pragma unroll
         for (int m = 0; m < 5; ++m) v[m] = ldo(f.q[m], qo);
 #pragma unroll
         for (int m = 0; m < 3; ++m) v[5 + m] = ldo(f.u[m], uo);
     };
-    if (has_outer) load_outer(z0 - 1, onext);
+    if (UNCOND || has_outer) load_outer(z0 - 1, onext);
 
     // z-difference values of the interior column's stage A at plane pa - 2
     // (the lower z neighbour of the plane finished in step 3), and the
@@ -502,14 +520,20 @@ __device__ __forceinline__ void fused_body(const Fields<T>& f, const BCDesc<T>& 
     for (int m = 0; m < kNumDZ; ++m) dz_back[m] = T(0);
     T fe_prev[3] = {0, 0, 0}, fe_cur[3] = {0, 0, 0};
 
-    for (int pa = z0 - 1; pa <= z1; ++pa) {
-        // ---- step 1: input plane pa into LDS; history advance ----
+    // One iteration: WB (compile time) false skips the gather and the force
+    // stage, for the two peeled planes of the unconditional form.
+    auto iteration = [&](const int pa, auto WB) {
+        const int pb = pa - 1;
+        const bool do_b = decltype(WB)::value && (UNCOND || (pb >= z0 && active));
+        if constexpr (!UNCOND) {
 #pragma unroll
-        for (int m = 0; m < kNumIn; ++m) {
-            hin[0][m] = hin[1][m];
-            hin[1][m] = hin[2][m];
-            hin[2][m] = hnext[m];
+            for (int m = 0; m < kNumIn; ++m) {
+                hin[0][m] = hin[1][m];
+                hin[1][m] = hin[2][m];
+                hin[2][m] = hnext[m];
+            }
         }
+        // ---- step 1: input plane pa into LDS; next-plane loads ----
         // At pa == z1 the loaded plane is never used; clamping it to z1 + 1
         // keeps every request within one plane past the chunk.
         const int pn = min(pa + 2, z1 + 1);
@@ -518,10 +542,9 @@ __device__ __forceinline__ void fused_body(const Fields<T>& f, const BCDesc<T>& 
         // so it is issued here and completes under the barrier and the Q
         // update. (The moving-wall term reads rho of the own cell, which only
         // this thread writes, after the collision below.)
-        const int pb = pa - 1;
         const unsigned cpb = (unsigned)pb * plane + (unsigned)gy * nx + gx;
         T pop[kQ];
-        if (pb >= z0 && active) gather32<PERIODIC>(f, bc, gx, gy, pb, cpb, pull, pop);
+        if (do_b) gather32<PERIODIC>(f, bc, gx, gy, pb, cpb, pull, pop);
         {
             const int cc = (ib + 1) * G::IX + ia + 1;
 #pragma unroll
@@ -535,14 +558,16 @@ __device__ __forceinline__ void fused_body(const Fields<T>& f, const BCDesc<T>& 
                 sRing[r_p][m][t] = rnext[m];
                 sIn[m][cc] = sRing[r_c][m][t];
             }
-            if (pa < z1) load_column<TX, TY>(f, qcr, ucr, qplane(pa + 2), uplane(pa + 2), rnext);
+            if (!UNCOND && pa < z1) load_column<TX, TY>(f, rq, ru, qplane(pa + 2), uplane(pa + 2), rnext);
         }
+        if (UNCOND) load_column<TX, TY>(f, rq, ru, qplane(pn), uplane(pn), rnext);
         if (has_outer) {
             const int cc = oj * G::IX + oi;
 #pragma unroll
             for (int m = 0; m < kNumIn; ++m) sIn[m][cc] = onext[m];
-            if (pa < z1) load_outer(pa + 1, onext);
+            if (!UNCOND && pa < z1) load_outer(pa + 1, onext);
         }
+        if (UNCOND) load_outer(min(pa + 1, z1), onext);
         __syncthreads();
 
         // ---- step 2: Q update (stage A) at plane pa ----
@@ -599,7 +624,7 @@ __device__ __forceinline__ void fused_body(const Fields<T>& f, const BCDesc<T>& 
         // ---- step 3: force + collide-stream at plane pb = pa - 1 ----
         const T* A0 = &sA[(pb + 2) & 1][0][0];
         auto A = [&](int m, int col) { return A0[m * G::NA + col]; };
-        if (pb >= z0 && active) {
+        if (do_b) {
             // Neighbour columns with the reference's fetch rule: where a
             // non-periodic face clamps, the cell itself.
             const int lxm = (qx0 == 0 && !ax[0].qlo) ? acell : acell - 1;
@@ -649,7 +674,7 @@ __device__ __forceinline__ void fused_body(const Fields<T>& f, const BCDesc<T>& 
             if constexpr (sizeof(T) == 4) collide_shifted(k, pop, F, drho, u);
             else collide_plain(k, pop, F, drho, u);
             const unsigned vs4 = (unsigned)f.vs * E, c4 = cpb * E;
-            if (wmask.writes(pb)) {
+            if (UNCOND || wmask.writes(pb)) {
 #pragma unroll
                 for (int j = 0; j < kQ; ++j) sto(f.gn, j * vs4 + c4, pop[j]);
 #pragma unroll
@@ -668,10 +693,26 @@ __device__ __forceinline__ void fused_body(const Fields<T>& f, const BCDesc<T>& 
 #pragma unroll
         for (int m = 0; m < kNumA; ++m)
             if (dz_slot(m) >= 0) dz_back[dz_slot(m)] = A(m, acell);
+        if constexpr (UNCOND) {
+            // History advance: planes pa, pa + 1, pa + 2 for the next iteration.
+#pragma unroll
+            for (int m = 0; m < kNumIn; ++m) {
+                hin[0][m] = hin[1][m];
+                hin[1][m] = hin[2][m];
+                hin[2][m] = hnext[m];
+            }
+        }
+    };
+    if constexpr (UNCOND) {
+        iteration(z0 - 1, std::false_type{});
+        iteration(z0, std::false_type{});
+        for (int pa = z0 + 1; pa <= z1; ++pa) iteration(pa, std::true_type{});
+    } else {
+        for (int pa = z0 - 1; pa <= z1; ++pa) iteration(pa, std::true_type{});
     }
 }
 
-template <int TX, int TY, int ZC, bool PERIODIC, typename T = float>
+template <int TX, int TY, int ZC, bool PERIODIC, typename T = float, bool UNCOND = false>
 __global__ void __launch_bounds__(TX * TY)
 fused_step_kernel(Fields<T> f, BCDesc<T> bc, Phys<T> k, int pull, int aux, int ntx, int nty,
                   int nzc, int mode) {
@@ -687,7 +728,7 @@ fused_step_kernel(Fields<T> f, BCDesc<T> bc, Phys<T> k, int pull, int aux, int n
     const int L = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
     if (L >= nblk) return;
     const int tx = L % ntx, ty = (L / ntx) % nty, tz = L / (ntx * nty);
-    fused_body<TX, TY, ZC, PERIODIC>(f, bc, k, pull, aux, tx, ty, tz, sIn, sA, sRing, mode);
+    fused_body<TX, TY, ZC, PERIODIC, T, UNCOND>(f, bc, k, pull, aux, tx, ty, tz, sIn, sA, sRing, mode);
 }
 
 // ---- wall shells of a single simulation, one thread per cell ------------------

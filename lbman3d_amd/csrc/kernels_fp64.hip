@@ -37,8 +37,14 @@ static void launch_tile64(const Fields<double>& f, const BCDesc<double>& bc, con
                           int aux, hipStream_t s) {
     const int ntx = (bc.n[0] + TX - 1) / TX, nty = (bc.n[1] + TY - 1) / TY, nzc = (bc.n[2] + ZC - 1) / ZC;
     const int grid = (ntx * nty * nzc + 7) / 8 * 8;
-    hipLaunchKernelGGL((fused_step_kernel<TX, TY, ZC, true, double>), dim3(grid), dim3(TX * TY), 0, s, f, bc, k, pull,
-                       aux, ntx, nty, nzc, 0);
+    // Grids of whole column tiles take the form with every memory
+    // instruction issued unconditionally.
+    if (bc.n[0] % TX == 0 && bc.n[1] % TY == 0)
+        hipLaunchKernelGGL((fused_step_kernel<TX, TY, ZC, true, double, true>), dim3(grid), dim3(TX * TY), 0, s, f,
+                           bc, k, pull, aux, ntx, nty, nzc, 0);
+    else
+        hipLaunchKernelGGL((fused_step_kernel<TX, TY, ZC, true, double>), dim3(grid), dim3(TX * TY), 0, s, f, bc, k,
+                           pull, aux, ntx, nty, nzc, 0);
 }
 
 void launch_fused_step_f64(const Fields<double>& f, const BCDesc<double>& bc, const Phys<double>& k, int tile,
