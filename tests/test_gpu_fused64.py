@@ -5,7 +5,9 @@ the Q-update results of the neighbouring columns on chip. It evaluates the
 same per-cell arithmetic as the two unfused kernels (no FMA contraction, the
 reference's operand order), so every tile shape must reproduce the two-kernel
 step bit for bit: rho, u, Q, the body force and the populations, on a grid
-whose sides are not multiples of any tile (partial tiles and z chunks), from
+whose sides are not multiples of any tile (partial tiles and z chunks), and
+on one whose x and y sides are whole tiles (the unconditional form, every
+load and store issued by every thread) with a partial last z chunk, from
 an uploaded state (own-slot first step) through several step calls. The
 two-kernel step is itself pinned to the oracle bit for bit by the fixture
 tests; the 32^3 1000-step and 256^3 tests run the fused step as the default
@@ -19,6 +21,9 @@ from lbman3d_amd import abi
 pytestmark = pytest.mark.gpu
 
 DIMS = (37, 22, 45)
+# x and y whole multiples of the 32x4, 16x8 and 32x8 tiles (their
+# unconditional form) with a partial last z chunk
+DIMS_WHOLE = (32, 16, 45)
 
 
 def run(cfg, q, sched):
@@ -35,12 +40,12 @@ def run(cfg, q, sched):
     return st
 
 
-@pytest.mark.parametrize("upwind", [False, True])
-def test_every_fp64_tile_equals_the_two_kernel_step(upwind):
+@pytest.mark.parametrize("dims,upwind", [(DIMS, False), (DIMS, True), (DIMS_WHOLE, True)])
+def test_every_fp64_tile_equals_the_two_kernel_step(dims, upwind):
     from lbman3d_amd.solver import DeviceSolver
-    cfg = abi.make_config(*DIMS, abi.FullyPeriodicConfig, abi.Params(ALPHA=0.046, LAMBDA=0.7 if upwind else 0.4),
+    cfg = abi.make_config(*dims, abi.FullyPeriodicConfig, abi.Params(ALPHA=0.046, LAMBDA=0.7 if upwind else 0.4),
                           advection=abi.ADV_UPWIND if upwind else abi.ADV_CENTRED, precision=abi.FP64)
-    q = abi.seed_director_field(31, *DIMS)
+    q = abi.seed_director_field(31, *dims)
     ref = run(cfg, q, 1)
     assert ref["name"] == "two-kernel"
     with DeviceSolver(cfg) as s:
@@ -54,7 +59,7 @@ def test_every_fp64_tile_equals_the_two_kernel_step(upwind):
     assert n >= 1
     for sched in [0] + list(range(2, 2 + n)):
         st = run(cfg, q, sched)
-        print(f"case FP64 {st['name']} upwind={upwind}", flush=True)
+        print(f"case FP64 {st['name']} dims={dims} upwind={upwind}", flush=True)
         if sched >= 2:
             assert st["name"].startswith("fused:fp64:")
         for k in ("rho", "u", "q", "F", "f"):
