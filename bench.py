@@ -282,7 +282,7 @@ def main():
             sched += 1
         s64.close()
         ms64 = ev64 / k64
-        k64name, roof64 = roofline(prof64, V, step_ms=ms64)
+        k64name, roof64 = roofline(prof64, V, name64.replace("auto:", "", 1), step_ms=ms64)
         secondary.update({"fp64_ms_per_step": round(ms64, 4), "fp64_mlups": round(V / (ms64 * 1e-3) / 1e6, 1),
                           "fp64_schedule": name64,
                           "fp64_step_hbm_frac": round(STEP_BYTES["fp64"] * V / (ms64 * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
