@@ -548,9 +548,11 @@ int lbm_batch_time_steps(lbm_batch_handle b, int32_t n, int32_t use_graph, doubl
     if (!b || !ms || n < 1) return fail(LBM_ERR_INVALID, "bad argument");
     if (b->desc_dirty)
         if (int r = rebuild(b)) return r;
-    hipEvent_t e0, e1;
-    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        if (e0) (void)hipEventDestroy(e0);
         return fail(LBM_ERR_HIP, "hipEventCreate failed");
+    }
     (void)hipEventRecord(e0, b->stream);
     int r = batch_steps(b, n, use_graph);
     (void)hipEventRecord(e1, b->stream);

@@ -47,6 +47,20 @@ int fail(int code, const std::string& msg) {
                                          __FILE__ + ":" + std::to_string(__LINE__) + ")");        \
     } while (0)
 
+namespace {
+// Runs a cleanup when the scope ends, so that the early returns of HIPCHK
+// release what a function has acquired so far.
+template <typename F>
+struct ScopeExit {
+    F f;
+    ~ScopeExit() { f(); }
+};
+template <typename F>
+ScopeExit<F> scope_exit(F f) {
+    return ScopeExit<F>{f};
+}
+}  // namespace
+
 // ---------------------------------------------------------------------------
 // configuration helpers
 // ---------------------------------------------------------------------------
@@ -910,7 +924,11 @@ int lbm_order_director_noalloc(lbm_handle s, double* order, double* director) {
 int lbm_time_steps(lbm_handle s, int32_t n, double* ms) {
     if (!s || !ms || n < 1) return fail(LBM_ERR_INVALID, "bad argument");
     ensure_tuned(s);
-    hipEvent_t a, b;
+    hipEvent_t a = nullptr, b = nullptr;
+    auto release = scope_exit([&] {
+        if (a) (void)hipEventDestroy(a);
+        if (b) (void)hipEventDestroy(b);
+    });
     HIPCHK(hipEventCreate(&a));
     HIPCHK(hipEventCreate(&b));
     HIPCHK(hipEventRecord(a, s->stream));
@@ -920,8 +938,6 @@ int lbm_time_steps(lbm_handle s, int32_t n, double* ms) {
     HIPCHK(hipEventSynchronize(b));
     float f = 0.0f;
     HIPCHK(hipEventElapsedTime(&f, a, b));
-    (void)hipEventDestroy(a);
-    (void)hipEventDestroy(b);
     *ms = f;
     return r;
 }
@@ -1085,14 +1101,19 @@ int lbm_bandwidth_probe(size_t bytes, int32_t reps, double* gbs, double* ms_per_
     if (!gbs || reps < 1 || bytes < 16) return fail(LBM_ERR_INVALID, "bad argument");
     bytes = bytes / 16 * 16;
     void *a = nullptr, *b = nullptr;
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    auto release = scope_exit([&] {
+        if (st) (void)hipStreamSynchronize(st);
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        if (st) (void)hipStreamDestroy(st);
+        if (a) (void)hipFree(a);
+        if (b) (void)hipFree(b);
+    });
     HIPCHK(hipMalloc(&a, bytes));
-    if (hipMalloc(&b, bytes) != hipSuccess) {
-        (void)hipFree(a);
-        return fail(LBM_ERR_HIP, "hipMalloc failed");
-    }
-    hipStream_t st;
+    HIPCHK(hipMalloc(&b, bytes));
     HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
-    hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
     HIPCHK(hipMemsetAsync(a, 0, bytes, st));
@@ -1105,11 +1126,6 @@ int lbm_bandwidth_probe(size_t bytes, int32_t reps, double* gbs, double* ms_per_
     HIPCHK(hipEventElapsedTime(&ms, e0, e1));
     *gbs = 2.0 * (double)bytes * reps / (ms * 1e-3) / 1e9;
     if (ms_per_copy) *ms_per_copy = ms / reps;
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    (void)hipStreamDestroy(st);
-    (void)hipFree(a);
-    (void)hipFree(b);
     return LBM_OK;
 }
 

@@ -537,6 +537,18 @@ namespace {
         if (e_ != hipSuccess) return fail(LBM_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
+// Runs a cleanup when the scope ends, so that the early returns of CHK
+// release what a function has acquired so far.
+template <typename F>
+struct ScopeExit {
+    F f;
+    ~ScopeExit() { f(); }
+};
+template <typename F>
+ScopeExit<F> scope_exit(F f) {
+    return ScopeExit<F>{f};
+}
+
 template <typename T>
 int cl_alloc(lbm_closure* c, T** p, size_t elems) {
     void* q = nullptr;
@@ -943,7 +955,11 @@ int lbm_closure_run(lbm_closure_handle c, lbm_handle sim, int32_t n, double alph
         return fail(LBM_ERR_INVALID, "closure and simulation grids differ");
     if (n == 0) return LBM_OK;
     ensure_tuned(sim);
-    hipEvent_t a, b;
+    hipEvent_t a = nullptr, b = nullptr;
+    auto release = scope_exit([&] {
+        if (a) (void)hipEventDestroy(a);
+        if (b) (void)hipEventDestroy(b);
+    });
     CHK(hipEventCreate(&a));
     CHK(hipEventCreate(&b));
     CHK(hipEventRecord(c->ev_before, c->stream));
@@ -960,15 +976,17 @@ int lbm_closure_run(lbm_closure_handle c, lbm_handle sim, int32_t n, double alph
     CHK(hipEventSynchronize(b));
     float f = 0.f;
     CHK(hipEventElapsedTime(&f, a, b));
-    (void)hipEventDestroy(a);
-    (void)hipEventDestroy(b);
     if (ms) *ms = f;
     return r;
 }
 
 int lbm_closure_time_forward(lbm_closure_handle c, int32_t n, double* ms) {
     if (!c || !ms || n < 1) return fail(LBM_ERR_INVALID, "bad argument");
-    hipEvent_t a, b;
+    hipEvent_t a = nullptr, b = nullptr;
+    auto release = scope_exit([&] {
+        if (a) (void)hipEventDestroy(a);
+        if (b) (void)hipEventDestroy(b);
+    });
     CHK(hipEventCreate(&a));
     CHK(hipEventCreate(&b));
     CHK(hipEventRecord(a, c->stream));
@@ -978,8 +996,6 @@ int lbm_closure_time_forward(lbm_closure_handle c, int32_t n, double* ms) {
     CHK(hipEventSynchronize(b));
     float f = 0.f;
     CHK(hipEventElapsedTime(&f, a, b));
-    (void)hipEventDestroy(a);
-    (void)hipEventDestroy(b);
     *ms = f / n;
     return LBM_OK;
 }
@@ -987,6 +1003,12 @@ int lbm_closure_time_forward(lbm_closure_handle c, int32_t n, double* ms) {
 int lbm_debug_mfma_bf16(const float* a, const float* b, float* d) {
     if (!a || !b || !d) return fail(LBM_ERR_INVALID, "null argument");
     float *da = nullptr, *db = nullptr, *dd = nullptr;
+    auto release = scope_exit([&] {
+        (void)hipDeviceSynchronize();
+        if (da) (void)hipFree(da);
+        if (db) (void)hipFree(db);
+        if (dd) (void)hipFree(dd);
+    });
     CHK(hipMalloc(&da, 16 * 32 * sizeof(float)));
     CHK(hipMalloc(&db, 32 * 16 * sizeof(float)));
     CHK(hipMalloc(&dd, 16 * 16 * sizeof(float)));
@@ -995,9 +1017,6 @@ int lbm_debug_mfma_bf16(const float* a, const float* b, float* d) {
     hipLaunchKernelGGL(mfma_check_kernel, dim3(1), dim3(64), 0, 0, da, db, dd);
     CHK(hipGetLastError());
     CHK(hipMemcpy(d, dd, 16 * 16 * sizeof(float), hipMemcpyDeviceToHost));
-    (void)hipFree(da);
-    (void)hipFree(db);
-    (void)hipFree(dd);
     return LBM_OK;
 }
 
