@@ -213,9 +213,11 @@ int lbm_tile_name(int32_t schedule, char* buf, size_t len);
 /* Name of a schedule of this handle's precision ("auto:...", "two-kernel",
  * "fused:..."). */
 int lbm_schedule_name(lbm_handle h, int32_t schedule, char* buf, size_t len);
-/* Time every schedule of the handle's precision for n steps on a scratch copy
- * of the state and keep the fastest for the automatic schedule; results per
- * schedule (1, 2, ...) in ms_out[count] (ms per step). */
+/* Time every schedule of the handle's precision for n steps from a snapshot
+ * of the state (restored after each trial), then the three fastest again for
+ * 8n steps each, and keep the fastest of those for the automatic schedule;
+ * results per schedule (1, 2, ...) in ms_out[count] (ms per step, the
+ * longer trial's for the three). */
 int lbm_autotune(lbm_handle h, int32_t n, double* ms_out, int32_t max, int32_t* count);
 
 /* HBM calibration: device-to-device copy of `bytes` (16-byte accesses),
