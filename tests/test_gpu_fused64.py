@@ -7,7 +7,8 @@ reference's operand order), so every tile shape must reproduce the two-kernel
 step bit for bit: rho, u, Q, the body force and the populations, on a grid
 whose sides are not multiples of any tile (partial tiles and z chunks), and
 on one whose x and y sides are whole tiles (the unconditional form, every
-load and store issued by every thread) with a partial last z chunk, from
+load and store issued by every thread) with a partial last z chunk, and on
+one- and two-plane grids of whole tiles, from
 an uploaded state (own-slot first step) through several step calls. The
 two-kernel step is itself pinned to the oracle bit for bit by the fixture
 tests; the 32^3 1000-step and 256^3 tests run the fused step as the default
@@ -40,7 +41,8 @@ def run(cfg, q, sched):
     return st
 
 
-@pytest.mark.parametrize("dims,upwind", [(DIMS, False), (DIMS, True), (DIMS_WHOLE, True)])
+@pytest.mark.parametrize("dims,upwind", [(DIMS, False), (DIMS, True), (DIMS_WHOLE, True), ((32, 16, 1), True),
+                                         ((32, 16, 2), False)])
 def test_every_fp64_tile_equals_the_two_kernel_step(dims, upwind):
     from lbman3d_amd.solver import DeviceSolver
     cfg = abi.make_config(*dims, abi.FullyPeriodicConfig, abi.Params(ALPHA=0.046, LAMBDA=0.7 if upwind else 0.4),
