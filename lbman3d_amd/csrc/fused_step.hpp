@@ -478,9 +478,8 @@ __device__ __forceinline__ void fused_body(const Fields<T>& f, const BCDesc<T>& 
     // Input history of the interior column in iteration pa: planes pa-1, pa,
     // pa+1 (registers); the next plane, pa + 2, is loaded one iteration ahead
     // of its use. It joins the history at the end of the iteration in the
-    // unconditional form (whose waits then fall after the stores are
-    // issued), at the start of the next one otherwise (measured faster
-    // there on the FP32 tiles).
+    // unconditional form (the wait for it then falls after the stores are
+    // issued) and at the start of the next iteration otherwise.
     static_assert(!UNCOND || PERIODIC, "unconditional form: fully periodic grids");
     T hin[3][kNumIn], hnext[kNumIn];
     load_column<TX, TY>(f, qc0, uc0, qplane(z0 - 2), uplane(z0 - 2), hin[UNCOND ? 0 : 1]);
